@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark: DOF-stage updates/s of the RKDG stage engine on the 1M-triangle p=2 tidal workload.
+
+Workload (one GPU): 708x708 quads split in 2 (1,002,528 triangles) over 708 km, interior
+vertices jittered +-20 %, depth 5-4000 m from 16 seeded Fourier modes, M2 tide on one side,
+Manning n = 0.025, p = 2 (18 DOF per triangle), SSP-RK3.  Inputs live in HBM before timing.
+
+A "step" is one SSP-RK3 step (3 stage launches).  value = DOF * stages / s of the headline
+form (FP32 bulk when its error against the FP64 form is within bound, else FP64 plain);
+both forms are reported.  roofline: algorithmic HBM bytes of one stage launch / mean
+stage-kernel time from HIP events on the context's stream, against 8 TB/s.
+cpu_baseline: the FP64 CPU oracle (OpenMP over host cores) on a bounded sample of the same
+recipe, timed on this host.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def dof_stages(ne, ndof, stages):
+    return ne * ndof * 3 * stages
+
+
+def time_form(mesh, prm, precision, steps, warmup):
+    from dgswemv2_amd import Simulation
+    sim = Simulation(mesh, prm, precision)
+    sim.step(warmup)
+    sim.sync()
+    sim.reset_timers()
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    wall = time.perf_counter() - t0
+    rep = sim.report()
+    q = sim.state()["q"]
+    sim.close()
+    return wall, rep, q
+
+
+def cpu_baseline(budget_s=12.0):
+    """FP64 oracle, OpenMP over the host cores, on a 71x71-quad sample of the tidal recipe."""
+    from dgswemv2_amd import workloads as W
+    from oracle.oracle_py import OracleSim, load
+    threads = max(1, min(16, os.cpu_count() or 1, load().orc_max_threads()))
+    mesh, prm = W.tidal(scale=0.1)
+    o = OracleSim(mesh, prm, threads=threads)
+    o.step(1)
+    n, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < budget_s:
+        o.step(1)
+        n += 1
+    el = time.perf_counter() - t0
+    v = dof_stages(mesh.n_elements, 6, 3 * n) / el
+    return {"value": v, "unit": "DOF-stage updates/s", "cores": threads, "kind": "port",
+            "sample": f"tidal recipe at 71x71 quads ({mesh.n_elements} triangles), p=2 SSP-RK3, {n} steps in "
+                      f"{el:.1f} s, FP64 oracle with OpenMP over {threads} threads (coloured interface loop)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--scale", type=float, default=1.0, help="mesh scale (1.0 = the 1M-triangle workload)")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine's multi-GPU sharding is "
+              "described in DESIGN.md and not built", file=sys.stderr)
+        sys.exit(2)
+
+    from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_PLAIN, workloads as W
+    import numpy as np
+
+    log("building the tidal mesh, scale", args.scale)
+    mesh, prm = W.tidal(scale=args.scale)
+    ne, ndof, ns = mesh.n_elements, 6, prm.nstages
+    log("triangles", ne, "dt", prm.dt)
+
+    forms = {}
+    qs = {}
+    for name, prec in (("fp64_plain", RKDG_FP64_PLAIN), ("fp32_bulk", RKDG_FP32_BULK)):
+        log("timing", name, "warmup", args.warmup, "steps", args.steps)
+        wall, rep, q = time_form(mesh, prm, prec, args.steps, args.warmup)
+        qs[name] = q
+        stages = args.steps * ns
+        kern_ms = rep["stage_kernel_ms"] / max(1, rep["stages_timed"])
+        achieved = rep["stage_bytes_alg"] / (kern_ms * 1e-3) / 1e9
+        forms[name] = {
+            "value": dof_stages(ne, ndof, stages) / wall,
+            "ms_per_step": wall / args.steps * 1e3,
+            "stage_kernel_ms": kern_ms,
+            "stage_bytes_alg": rep["stage_bytes_alg"],
+            "achieved_GBs": achieved,
+            "roofline_frac": achieved / HBM_PEAK_GBS,
+        }
+        log(name, json.dumps(forms[name]))
+    # FP32-bulk error against the FP64 form after the same steps (same start state)
+    rel = []
+    for k in range(3):
+        a, b = qs["fp32_bulk"][k], qs["fp64_plain"][k]
+        rel.append(float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)))
+    forms["fp32_bulk"]["rel_l2_vs_fp64"] = rel
+    head = "fp32_bulk" if max(rel) <= 1e-6 else "fp64_plain"
+    h = forms[head]
+
+    traffic = None
+    tf = ROOT / "profiles" / f"traffic_{head}.json"
+    if tf.exists():
+        try:
+            traffic = json.loads(tf.read_text()).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None if args.no_cpu else cpu_baseline()
+    if cpu:
+        log("cpu baseline", json.dumps(cpu))
+
+    out = {
+        "metric": "DOF-stage updates/s at p=2, % HBM roofline; mixed+learned vs FP64 plain",
+        "value": h["value"],
+        "unit": "DOF-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": h["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if head == "fp32_bulk" else "f64",
+        "data": "synthetic",
+        "config": {"workload": f"tidal p=2 SSP-RK3, {ne} triangles (708x708 quads, jitter 20%, Fourier depth "
+                               f"5-4000 m, M2 tide, Manning 0.025)", "triangles": ne, "p": 2, "stages": ns,
+                   "form": head},
+        "forms": forms,
+        "fp32_over_fp64": forms["fp32_bulk"]["value"] / forms["fp64_plain"]["value"],
+        "roofline": {"bound": "hbm", "achieved": h["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": h["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,230 @@
+"""MI355X RKDG shallow-water stage engine: Python host side.
+
+Mirrors the reference's driver surface (Serial::Simulation<SWE::RKDG::Problem>,
+source/simulation/serial/simulation.hpp:14-98): a Simulation is built from a mesh and the
+problem inputs, steps with SSP-RK (Run/step_serial), exposes the modal state and reports the
+L2 residual (ComputeL2Residual).  All numerics run in librkdg.so (HIP kernels + C++ host);
+this module only marshals arrays through the C-ABI of include/rkdg.h.
+"""
+import ctypes as C
+import math
+
+import numpy as np
+
+from . import _native as N
+from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_TIDE, FRICTION_CHEZY, FRICTION_MANNING,
+                      FRICTION_NONE, PROBLEM_DAM_BREAK, PROBLEM_MANUFACTURED, PROBLEM_NONE,
+                      PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_PLAIN, RkdgError)
+
+__all__ = [
+    "Mesh", "quad_mesh", "make_params", "Simulation", "RkdgError", "ndof_of",
+    "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION",
+    "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
+    "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK",
+]
+
+
+def ndof_of(p):
+    return (p + 1) * (p + 2) // 2
+
+
+class Mesh:
+    """ADCIRC-style mesh: nodes (x, y, depth), triangles, typed boundary edges, forcing."""
+
+    def __init__(self, node_xyz, elt_nodes, bound_edges, bound_types, frequency=(), forcing_fact=(),
+                 equilib_arg=(), node_amplitude=None, node_phase=None, node_manning_n=None, _owner=None):
+        self.node_xyz = np.ascontiguousarray(node_xyz, dtype=np.float64).reshape(-1, 3)
+        self.elt_nodes = np.ascontiguousarray(elt_nodes, dtype=np.int32).reshape(-1, 3)
+        self.bound_edges = np.ascontiguousarray(bound_edges, dtype=np.int32).reshape(-1, 2)
+        self.bound_types = np.ascontiguousarray(bound_types, dtype=np.uint8).reshape(-1)
+        self.frequency = np.ascontiguousarray(frequency, dtype=np.float64)
+        self.forcing_fact = np.ascontiguousarray(forcing_fact, dtype=np.float64)
+        self.equilib_arg = np.ascontiguousarray(equilib_arg, dtype=np.float64)
+        nc = len(self.frequency)
+        self.node_amplitude = None if node_amplitude is None else np.ascontiguousarray(node_amplitude, np.float64)
+        self.node_phase = None if node_phase is None else np.ascontiguousarray(node_phase, np.float64)
+        self.node_manning_n = None if node_manning_n is None else np.ascontiguousarray(node_manning_n, np.float64)
+        if nc and (self.node_amplitude is None or self.node_phase is None):
+            raise ValueError("constituents given without nodal amplitude/phase")
+        self._owner = _owner
+        self.desc = N.MeshDesc()
+        d = self.desc
+        d.n_nodes, d.n_elements = len(self.node_xyz), len(self.elt_nodes)
+        d.node_xyz, d.elt_nodes = N.dptr(self.node_xyz), N.iptr(self.elt_nodes)
+        d.n_bound_edges = len(self.bound_types)
+        d.bound_edge_nodes, d.bound_edge_type = N.iptr(self.bound_edges), N.uptr(self.bound_types)
+        d.n_constituents = nc
+        d.frequency = N.dptr(self.frequency) if nc else None
+        d.forcing_fact = N.dptr(self.forcing_fact) if nc else None
+        d.equilib_arg = N.dptr(self.equilib_arg) if nc else None
+        d.node_amplitude = N.dptr(self.node_amplitude)
+        d.node_phase = N.dptr(self.node_phase)
+        d.node_manning_n = N.dptr(self.node_manning_n)
+
+    @property
+    def n_elements(self):
+        return len(self.elt_nodes)
+
+    def min_edge_length(self):
+        x, y = self.node_xyz[:, 0], self.node_xyz[:, 1]
+        e = self.elt_nodes
+        lens = [np.hypot(x[e[:, a]] - x[e[:, b]], y[e[:, a]] - y[e[:, b]]) for a, b in ((0, 1), (1, 2), (2, 0))]
+        return float(min(l.min() for l in lens))
+
+
+def quad_mesh(x1, y1, x2, y2, m, n, bc=(BC_LAND,) * 4, pattern=0, bathy="constant", b0=1.0, alpha=1e-7,
+              bmin=5.0, bmax=4000.0, n_modes=16, jitter=0.0, seed=0, step=(0.0, 1.0, 1.0), frequency=0.0,
+              forcing_fact=1.0, equilib_arg=0.0, amplitude=1.0, phase=0.0, manning_n=0.0):
+    """Quad-split triangle mesh from the library's generator (quad_mesh_generator semantics).
+
+    bc: codes of the sides y=y1, x=x2, y=y2, x=x1 (reference order bottom, right, top, left).
+    bathy: "constant" (b0), "bowl" (-alpha (x^2+y^2)), "fourier" (seeded modes clamped to
+    [bmin, bmax]) or "step" (step=(x_step, depth_left, depth_right)).
+    """
+    lib = N.load_library()
+    sp = N.QuadMeshSpec()
+    sp.x1, sp.y1, sp.x2, sp.y2 = x1, y1, x2, y2
+    sp.m, sp.n, sp.pattern = m, n, pattern
+    for i in range(4):
+        sp.bc[i] = bc[i]
+    sp.bathy_kind = {"constant": 0, "bowl": 1, "fourier": 2, "step": 3}[bathy]
+    sp.b0, sp.alpha, sp.bmin, sp.bmax, sp.n_modes = b0, alpha, bmin, bmax, n_modes
+    sp.jitter, sp.seed = jitter, seed
+    sp.step_x, sp.step_b_left, sp.step_b_right = step
+    sp.frequency, sp.forcing_fact, sp.equilib_arg = frequency, forcing_fact, equilib_arg
+    sp.amplitude, sp.phase, sp.manning_n = amplitude, phase, manning_n
+    handle = C.c_void_p()
+    desc = N.MeshDesc()
+    st = lib.rkdg_mesh_generate(C.byref(sp), C.byref(handle), C.byref(desc))
+    if st != N.RKDG_OK:
+        raise RkdgError(st, "mesh generation failed")
+    try:
+        nn, ne = desc.n_nodes, desc.n_elements
+        xyz = np.ctypeslib.as_array(desc.node_xyz, shape=(nn * 3,)).copy()
+        elt = np.ctypeslib.as_array(desc.elt_nodes, shape=(ne * 3,)).copy()
+        nb = desc.n_bound_edges
+        be = np.ctypeslib.as_array(desc.bound_edge_nodes, shape=(nb * 2,)).copy()
+        bt = np.ctypeslib.as_array(desc.bound_edge_type, shape=(nb,)).copy()
+        nc = desc.n_constituents
+        kw = {}
+        if nc:
+            kw = dict(frequency=np.ctypeslib.as_array(desc.frequency, shape=(nc,)).copy(),
+                      forcing_fact=np.ctypeslib.as_array(desc.forcing_fact, shape=(nc,)).copy(),
+                      equilib_arg=np.ctypeslib.as_array(desc.equilib_arg, shape=(nc,)).copy(),
+                      node_amplitude=np.ctypeslib.as_array(desc.node_amplitude, shape=(nn * nc,)).copy(),
+                      node_phase=np.ctypeslib.as_array(desc.node_phase, shape=(nn * nc,)).copy())
+        if desc.node_manning_n:
+            kw["node_manning_n"] = np.ctypeslib.as_array(desc.node_manning_n, shape=(nn,)).copy()
+    finally:
+        lib.rkdg_mesh_free(handle)
+    return Mesh(xyz, elt, be, bt, **kw)
+
+
+def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, problem=PROBLEM_NONE,
+                ic_constant=True, ze0=0.0, qx0=0.0, qy0=0.0, function_source=False, friction=FRICTION_NONE,
+                friction_cf=0.0, wetting_drying=False, h_o=0.1, slope_limiting=False, sl_M=1.0e-8, sl_nu=1.5,
+                problem_param=(0.0, 0.0, 0.0, 0.0)):
+    """rkdg_params with the reference's input defaults (problem_input/swe_inputs.hpp:90-155)."""
+    prm = N.Params()
+    prm.p, prm.nstages, prm.order, prm.dt, prm.g, prm.ramp_duration = p, nstages, order, dt, g, ramp_duration
+    prm.problem, prm.ic_constant = problem, int(bool(ic_constant))
+    prm.ze0, prm.qx0, prm.qy0 = ze0, qx0, qy0
+    prm.function_source, prm.friction, prm.friction_cf = int(bool(function_source)), friction, friction_cf
+    prm.wetting_drying, prm.h_o = int(bool(wetting_drying)), h_o
+    prm.slope_limiting, prm.sl_M, prm.sl_nu = int(bool(slope_limiting)), sl_M, sl_nu
+    for i in range(4):
+        prm.problem_param[i] = problem_param[i]
+    return prm
+
+
+class Simulation:
+    """One mesh + problem on the GPU (the reference's Serial::Simulation)."""
+
+    def __init__(self, mesh, params, precision=RKDG_FP64_PLAIN):
+        self.lib = N.load_library()
+        self.mesh, self.params, self.precision = mesh, params, precision
+        self.ne = mesh.n_elements
+        self.ndof = ndof_of(params.p)
+        h = C.c_void_p()
+        st = self.lib.rkdg_create(C.byref(mesh.desc), C.byref(params), precision, C.byref(h))
+        if st != N.RKDG_OK:
+            raise RkdgError(st, self.lib.rkdg_create_error().decode())
+        self.ctx = h
+
+    def _check(self, st):
+        if st != N.RKDG_OK:
+            raise RkdgError(st, self.lib.rkdg_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.rkdg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def step(self, nsteps=1):
+        self._check(self.lib.rkdg_step(self.ctx, nsteps))
+
+    def stage(self):
+        self._check(self.lib.rkdg_stage(self.ctx))
+
+    def sync(self):
+        self._check(self.lib.rkdg_sync(self.ctx))
+
+    def state(self):
+        """Modal state [3, ndof, ne] plus wet / limited [3, ne] / troubled flags."""
+        q = np.empty(3 * self.ndof * self.ne)
+        wet = np.empty(self.ne, np.uint8)
+        lim = np.empty(3 * self.ne, np.uint8)
+        tr = np.empty(self.ne, np.uint8)
+        self._check(self.lib.rkdg_get_state(self.ctx, N.dptr(q), N.uptr(wet), N.uptr(lim), N.uptr(tr)))
+        return {"q": q.reshape(3, self.ndof, self.ne), "wet": wet, "limited": lim.reshape(3, self.ne),
+                "troubled": tr}
+
+    def set_state(self, q, wet=None):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        w = None if wet is None else np.ascontiguousarray(wet, dtype=np.uint8)
+        self._check(self.lib.rkdg_set_state(self.ctx, N.dptr(q), N.uptr(w)))
+
+    def time(self):
+        t = C.c_double()
+        step, stage = C.c_int32(), C.c_int32()
+        self._check(self.lib.rkdg_get_time(self.ctx, C.byref(t), C.byref(step), C.byref(stage)))
+        return t.value, step.value, stage.value
+
+    def set_time(self, t, step):
+        self._check(self.lib.rkdg_set_time(self.ctx, t, step))
+
+    def nan_info(self):
+        e, s, k = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.lib.rkdg_nan_info(self.ctx, C.byref(e), C.byref(s), C.byref(k)))
+        return e.value, s.value, k.value
+
+    def l2_residual(self):
+        v = C.c_double()
+        self._check(self.lib.rkdg_l2_residual(self.ctx, C.byref(v)))
+        return v.value
+
+    def reset_timers(self):
+        self._check(self.lib.rkdg_reset_timers(self.ctx))
+
+    def report(self):
+        r = N.PerfReport()
+        self._check(self.lib.rkdg_report(self.ctx, C.byref(r)))
+        return {f: getattr(r, f) for f, _ in N.PerfReport._fields_}
+
+    def run(self, run_time):
+        """Simulation::Run: ceil(run_time / dt) full steps (simulation.hpp:47, :68-82)."""
+        n = int(math.ceil(run_time / self.params.dt))
+        self.step(n)
+        return n

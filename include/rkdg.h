@@ -1,0 +1,223 @@
+/*
+ * rkdg.h -- C-ABI of the MI355X RKDG shallow-water stage engine (librkdg.so).
+ *
+ * The reference exposes this path as a compile-time C++ "Problem" concept, not as an
+ * FFI: SWE::RKDG::Problem (source/problem/SWE/discretization_RKDG/rkdg_swe_problem.hpp:34-194)
+ * driven by Serial::Simulation<P>::Run (source/simulation/serial/simulation.hpp:68-82).
+ * Each entry point below names the reference hook(s) it replaces.  All calls are
+ * stream-ordered on the context's own HIP stream; a context is owned by one host thread
+ * (the reference's one-owner-per-submesh rule, simulation/ompi/simulation_ompi.hpp:63).
+ * No C++ exceptions cross this boundary: every call returns an rkdg_status and the
+ * message of the last failure is available from rkdg_last_error().
+ *
+ * Layout conventions (host arrays passed in and out):
+ *   modal state q is [3][ndof][n_elements], variable-major then mode then element
+ *   (ze = 0, qx = 1, qy = 2; modes in the reference's Dubiner order, basis_dubiner_2D.cpp:18-23);
+ *   flags are one byte per element.
+ */
+#ifndef RKDG_H
+#define RKDG_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rkdg_ctx rkdg_ctx;
+
+typedef enum {
+    RKDG_OK = 0,
+    RKDG_ERR_ARG = 1,         /* invalid argument / mesh (reference: std::logic_error at setup) */
+    RKDG_ERR_HIP = 2,         /* a HIP runtime call failed */
+    RKDG_ERR_NAN = 3,         /* NaN in the state (reference: scrutinize_solution -> abort()) */
+    RKDG_ERR_UNSUPPORTED = 4, /* configuration not built (e.g. p outside 1..2 on the device) */
+    RKDG_ERR_NO_DEVICE = 5    /* no HIP device visible */
+} rkdg_status;
+
+typedef enum {
+    RKDG_FP64_PLAIN = 0, /* FP64 everywhere: parity form */
+    RKDG_FP32_BULK = 1   /* FP32 state/flux/accumulation; FP64 geometry precompute and wet/dry */
+} rkdg_precision;
+
+/* Boundary condition codes (reference: SWE::BoundaryTypes, problem/SWE/swe_definitions.hpp:61-69). */
+enum {
+    RKDG_BC_LAND = 0,
+    RKDG_BC_TIDE = 1,
+    RKDG_BC_FLOW = 2,
+    RKDG_BC_FUNCTION = 3
+};
+
+/* Closed-form problem function sets (reference: compile-time ic_q/source_q/true_q files,
+ * problem/SWE/problem_function_files and examples/; here a runtime choice). */
+enum {
+    RKDG_PROBLEM_NONE = 0,          /* constant initial state, no function source, no truth */
+    RKDG_PROBLEM_MANUFACTURED = 1,  /* examples/swe_manufactured_solution */
+    RKDG_PROBLEM_PARABOLIC_BOWL = 2,/* examples/swe_parabolic_bowl */
+    RKDG_PROBLEM_DAM_BREAK = 3      /* step in free surface at x = x0 (problem_param[0..2]) */
+};
+
+enum { RKDG_FRICTION_NONE = 0, RKDG_FRICTION_CHEZY = 1, RKDG_FRICTION_MANNING = 2 };
+
+/*
+ * Mesh description, in the reference's ADCIRC model (preprocessor/ADCIRC_reader/adcirc_format.cpp:3-…):
+ * nodes carry (x, y, depth), depth positive below the datum; element e is row e of
+ * elt_nodes and its ID is e (elements iterate in ascending ID, heterogeneous_containers.hpp:76-78).
+ * Every mesh edge with one adjacent element must be listed in bound_edge_nodes with its
+ * boundary code (reference: AdcircFormat::get_ibtype, adcirc_format.cpp:214).
+ * Tide/flow forcing: n_constituents harmonic constituents shared by all forced nodes
+ * (reference .bcis data, problem_input/swe_inputs.cpp:212-…), amplitude/phase per node.
+ */
+typedef struct rkdg_mesh_desc {
+    int32_t n_nodes;
+    int32_t n_elements;
+    const double* node_xyz;          /* [n_nodes*3] */
+    const int32_t* elt_nodes;        /* [n_elements*3] */
+    int32_t n_bound_edges;
+    const int32_t* bound_edge_nodes; /* [n_bound_edges*2] */
+    const uint8_t* bound_edge_type;  /* [n_bound_edges], RKDG_BC_* */
+    int32_t n_constituents;
+    const double* frequency;         /* [n_constituents] rad/s */
+    const double* forcing_fact;      /* [n_constituents] */
+    const double* equilib_arg;       /* [n_constituents] degrees */
+    const double* node_amplitude;    /* [n_nodes*n_constituents] or NULL */
+    const double* node_phase;        /* [n_nodes*n_constituents] degrees, or NULL */
+    const double* node_manning_n;    /* [n_nodes] or NULL */
+} rkdg_mesh_desc;
+
+/* Run parameters (reference: InputParameters + SWE::Inputs, preprocessor/input_parameters.hpp:96-261,
+ * problem_input/swe_inputs.hpp:126-155, and the globals set by
+ * problem_preprocessor/swe_pre_init_prob_params.hpp:5-47). */
+typedef struct rkdg_params {
+    int32_t p;               /* polynomial order */
+    int32_t nstages;         /* SSP-RK stages */
+    int32_t order;           /* SSP-RK order */
+    double dt;
+    double g;
+    double ramp_duration;    /* days; 0 = no ramp */
+    int32_t problem;         /* RKDG_PROBLEM_* : initial condition / function BC / truth */
+    int32_t ic_constant;     /* 1: constant IC (ze0,qx0,qy0) projected nodally; 0: ic_q(0,x) */
+    double ze0, qx0, qy0;
+    int32_t function_source; /* 1: add source_q(t,x) (manufactured) */
+    int32_t friction;        /* RKDG_FRICTION_* */
+    double friction_cf;      /* Chezy coefficient / Manning floor */
+    int32_t wetting_drying;
+    double h_o;
+    int32_t slope_limiting;  /* Cockburn-Shu */
+    double sl_M;
+    double sl_nu;
+    double problem_param[4]; /* problem-specific constants (dam break: x0, h_left, h_right) */
+} rkdg_params;
+
+typedef struct rkdg_perf_report {
+    int64_t n_elements;
+    int32_t p, ndof, nstages;
+    int32_t precision;
+    int64_t stages_timed;        /* stages executed since the last rkdg_reset_timers */
+    double stage_kernel_ms;      /* summed HIP-event time of the fused stage kernel */
+    double limiter_kernel_ms;    /* summed HIP-event time of the limiter kernels */
+    double total_ms;             /* HIP-event time across whole rkdg_step calls */
+    double stage_bytes_alg;      /* algorithmic HBM bytes per stage launch */
+    int32_t regs_per_thread;     /* unused (0) unless filled by a profiler */
+} rkdg_perf_report;
+
+/* ---- context lifecycle ---------------------------------------------------------- */
+
+/* Builds mesh connectivity (interfaces, boundaries), FP64 geometry, master tables and the
+ * initial state, then uploads everything to device HBM.
+ * Replaces: initialize_mesh (preprocessor/initialize_mesh.hpp:19-77),
+ *   SWE::RKDG::Problem::create_interfaces/create_boundaries (rkdg_swe_problem.hpp:57-113),
+ *   preprocessor_serial + initialize_data_serial (kernels_preprocessor/rkdg_swe_pre_serial.hpp:8-16,
+ *   problem_preprocessor/swe_pre_init_data.hpp:8-327) and the ESSPRKStepper ctor
+ *   (simulation/stepper/explicit_ssp_rk_stepper.cpp:3-14). */
+rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, rkdg_precision precision,
+                        rkdg_ctx** out);
+void rkdg_destroy(rkdg_ctx* ctx);
+const char* rkdg_last_error(const rkdg_ctx* ctx);
+/* Message of the last rkdg_create failure (no context exists then). */
+const char* rkdg_create_error(void);
+
+/* ---- state ------------------------------------------------------------------------ */
+
+/* Overwrites state[0].q (and the wet flags if wet != NULL); replaces direct writes into
+ * SWE::Data::state (problem_data_structure/swe_data.hpp:13-70). */
+rkdg_status rkdg_set_state(rkdg_ctx* ctx, const double* q, const uint8_t* wet);
+/* Reads the current-stage state.  limited: [3][n_elements] per-variable slope-limiter
+ * write-back flags of the last limiter pass (swe_CS_slope_limiter.hpp:180-187);
+ * troubled: slope_limit_state.troubled (swe_trouble_check.hpp:106-115).  Any pointer may be NULL. */
+rkdg_status rkdg_get_state(rkdg_ctx* ctx, double* q, uint8_t* wet, uint8_t* limited, uint8_t* troubled);
+
+/* ---- time stepping ---------------------------------------------------------------- */
+
+/* nsteps full SSP-RK steps (Problem::step_serial, kernels_processor/rkdg_swe_proc_serial_step.hpp:9-26),
+ * each stage = volume, source, interfaces, boundaries, M^-1 + update, ++stepper, wetting-drying,
+ * slope limiter, NaN scan (stage_serial, :28-67).  Returns RKDG_ERR_NAN if a NaN was found;
+ * rkdg_nan_info then names where. */
+rkdg_status rkdg_step(rkdg_ctx* ctx, int32_t nsteps);
+/* One stage (stage_serial), for per-stage parity checks. */
+rkdg_status rkdg_stage(rkdg_ctx* ctx);
+/* Stepper bookkeeping: t, step, stage (ESSPRKStepper, explicit_ssp_rk_stepper.hpp:42-47). */
+rkdg_status rkdg_get_time(rkdg_ctx* ctx, double* t, int32_t* step, int32_t* stage);
+/* Sets the stepper's time and step counter at a step boundary (stage 0), for restarts from a
+ * saved state; no reference counterpart (the reference has no resume, its modal output
+ * swe_post_write_modal.hpp:5-90 has no reader). */
+rkdg_status rkdg_set_time(rkdg_ctx* ctx, double t, int32_t step);
+/* First NaN found: element ID, step and stage (scrutinize_solution, swe_post_scrutinize.hpp:5-41). */
+rkdg_status rkdg_nan_info(rkdg_ctx* ctx, int32_t* element, int32_t* step, int32_t* stage);
+/* Waits for all work queued on the context's stream. */
+rkdg_status rkdg_sync(rkdg_ctx* ctx);
+
+/* ---- readouts --------------------------------------------------------------------- */
+
+/* sqrt(sum_e int_e |true_q(t) - q|^2) on Dunavant rule 20 at the current stage time.
+ * Replaces Simulation::ComputeL2Residual (simulation/serial/simulation.hpp:85-93) and
+ * Element::ComputeResidualL2 (geometry/element.hpp:547-592).  Host-side FP64. */
+rkdg_status rkdg_l2_residual(rkdg_ctx* ctx, double* l2);
+/* Per-kernel HIP-event times and the algorithmic byte count of one stage launch. */
+rkdg_status rkdg_report(rkdg_ctx* ctx, rkdg_perf_report* report);
+rkdg_status rkdg_reset_timers(rkdg_ctx* ctx);
+
+/* ---- host-side helpers (no device work) ------------------------------------------- */
+
+/* Reference-element tables of the device path for order p (phi, dphi at the rule-2p points,
+ * Dunavant weights), written into caller arrays sized [ndof*ngp] / [ngp].  Lets tests compare
+ * the engine's basis against the oracle bit for bit (Master::Triangle ctor, master_triangle.tpp:5-87). */
+rkdg_status rkdg_master_tables(int32_t p, int32_t* ndof, int32_t* ngp, double* phi, double* dphi_z1,
+                               double* dphi_z2, double* w);
+
+/* Shu-Osher coefficients ark/brk ([nstages][nstages], zero outside the scheme) and stage-time
+ * offsets drk of an explicit SSP-RK scheme (ESSPRKStepper::InitializeCoefficients,
+ * simulation/stepper/explicit_ssp_rk_stepper.cpp:16-276). */
+rkdg_status rkdg_ssprk_coeffs(int32_t nstages, int32_t order, double* ark, double* brk, double* drk);
+
+/* Structured quad-split triangle mesh over a rectangle, the reference's quad_mesh_generator
+ * (mesh_generators/quad_mesh_generator.cpp:91-279, patterns :464-541) extended with seeded
+ * vertex jitter and seeded Fourier-mode bathymetry.  Coordinates are rounded through "%.12f"
+ * text exactly as a written/read .14 file would be (quad_mesh_generator.cpp:164). */
+typedef struct rkdg_quad_mesh_spec {
+    double x1, y1, x2, y2;      /* rectangle corners (lower-left, upper-right) */
+    int32_t m, n;               /* quads in x, y */
+    int32_t pattern;            /* 0 simple, 1 zigzag, 2 checker */
+    int32_t bc[4];              /* RKDG_BC_* for sides: y=y1, x=x2, y=y2, x=x1 */
+    int32_t bathy_kind;         /* 0 constant(b0), 1 bowl -alpha*(x^2+y^2), 2 Fourier modes, 3 step */
+    double b0, alpha;           /* constant depth / bowl coefficient */
+    double bmin, bmax;          /* Fourier clamp range */
+    int32_t n_modes;            /* Fourier modes */
+    double jitter;              /* interior vertex jitter as a fraction of spacing */
+    uint64_t seed;
+    double step_x, step_b_left, step_b_right; /* bathy_kind 3: depth b_left for x<step_x else b_right */
+    /* one tidal/flow constituent for tide/flow sides */
+    double frequency, forcing_fact, equilib_arg, amplitude, phase;
+    double manning_n;           /* >0: uniform nodal Manning n */
+} rkdg_quad_mesh_spec;
+
+typedef struct rkdg_mesh rkdg_mesh; /* owns the arrays a rkdg_mesh_desc points into */
+rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* spec, rkdg_mesh** out, rkdg_mesh_desc* desc);
+void rkdg_mesh_free(rkdg_mesh* mesh);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RKDG_H */

@@ -1,0 +1,1664 @@
+// oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// FP64 CPU restatement of the dgswemv2 RKDG shallow-water stage path, used as the parity
+// checker for the HIP engine (and, timed, as the CPU baseline).  Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg load it; the engine never does.
+//
+// Every routine follows the reference's loop order and arithmetic order (operand order of
+// each +,-,*,/; sums over quadrature points / modes run left to right starting from the
+// first product, as Eigen 3.3's coefficient-based product does for these small sizes).
+// Build with -ffp-contract=off: the reference's default Release build on x86-64 emits no FMA.
+//
+// Parity pins (see tests/): Dubiner phi/dphi against the reference's p=10 golden vectors
+// (test/test_basis_dubiner_2d.cpp), quadrature exactness (test_integ_*), LLF golden vectors
+// (test_llf_flux.cpp), element integration golden vectors (test_element_triangle.hpp),
+// the five wetting-drying scenarios (test_wetting_drying.cpp), the RK ODE test
+// (test_rk_stepper.cpp), plus invariants (lake at rest, mass conservation, manufactured-
+// solution convergence, limiter invariants).  The whole-run trajectories cannot be pinned
+// against the reference binary (it does not build here); see the test headers.
+//
+// Third-party arithmetic restated: Eigen 3.3.4 (not vendored in the reference) 2x2
+// determinant a00*a11 - a10*a01 and inverse (1/det then cofactor*invdet), and the
+// coefficient-based small-matrix product order noted above.
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <set>
+#include <string>
+#include <unordered_map>
+#include <vector>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#include "../include/rkdg.h"
+#include "../dgswemv2_amd/csrc/quadrature_data.h"
+
+namespace orc {
+
+constexpr double PI = 3.14159265359;  // general_definitions.hpp:227
+constexpr unsigned DEFAULT_ID = 4294967295u;
+
+// ----------------------------------------------------------------------------------------
+// Small row-major dense matrix.
+struct Mat {
+    int r = 0, c = 0;
+    std::vector<double> a;
+    Mat() = default;
+    Mat(int r_, int c_, double v = 0.0) : r(r_), c(c_), a((size_t)r_ * c_, v) {}
+    double& operator()(int i, int j) { return a[(size_t)i * c + j]; }
+    double operator()(int i, int j) const { return a[(size_t)i * c + j]; }
+};
+
+// C = A * B, each coefficient summed over k from k = 0 (Eigen lazy product order).
+static Mat mul(const Mat& A, const Mat& B) {
+    Mat C(A.r, B.c);
+    for (int i = 0; i < A.r; ++i)
+        for (int j = 0; j < B.c; ++j) {
+            double s = A(i, 0) * B(0, j);
+            for (int k = 1; k < A.c; ++k) s = s + A(i, k) * B(k, j);
+            C(i, j) = s;
+        }
+    return C;
+}
+
+// ----------------------------------------------------------------------------------------
+// Jacobi polynomials (basis/polynomials/basis_polynomials.cpp:4-67).
+static std::vector<double> jacobi(unsigned n, unsigned a, unsigned b, const std::vector<double>& x) {
+    size_t npt = x.size();
+    std::vector<double> P(npt);
+    if (n == 0) {
+        for (size_t k = 0; k < npt; ++k) P[k] = 1;
+    } else if (n == 1) {
+        for (size_t k = 0; k < npt; ++k) P[k] = ((double)(a - b) + (double)(a + b + 2) * x[k]) / 2;
+    } else {
+        std::vector<double> J0(npt, 1.0), J1(npt), J2(npt, 0.0);
+        std::vector<double>* J[3] = {&J0, &J1, &J2};
+        for (size_t k = 0; k < npt; ++k) J1[k] = ((double)(a - b) + (double)(a + b + 2) * x[k]) / 2;
+        for (unsigned i = 1; i < n; ++i) {
+            double a1 = (double)(2 * (i + 1) * (i + a + b + 1) * (2 * i + a + b));
+            double a2 = (double)((2 * i + a + b + 1) * (a * a - b * b));
+            double a3 = (double)((2 * i + a + b) * (2 * i + a + b + 1) * (2 * i + a + b + 2));
+            double a4 = (double)(2 * (i + a) * (i + b) * (2 * i + a + b + 2));
+            std::vector<double>& Jn = *J[(i + 1) % 3];
+            std::vector<double>& Ji = *J[i % 3];
+            std::vector<double>& Jm = *J[(i - 1) % 3];
+            for (size_t k = 0; k < npt; ++k) Jn[k] = ((a2 + a3 * x[k]) * Ji[k] - a4 * Jm[k]) / a1;
+        }
+        for (size_t k = 0; k < npt; ++k) P[k] = (*J[n % 3])[k];
+    }
+    return P;
+}
+
+static std::vector<double> jacobi_derivative(unsigned n, unsigned a, unsigned b, const std::vector<double>& x) {
+    size_t npt = x.size();
+    std::vector<double> dP(npt);
+    if (n == 0) {
+        for (size_t k = 0; k < npt; ++k) dP[k] = 0.0;
+    } else if (n == 1) {
+        for (size_t k = 0; k < npt; ++k) dP[k] = (a + b + n + 1) / 2.0;
+    } else {
+        dP = jacobi(n - 1, a + 1, b + 1, x);
+        for (size_t k = 0; k < npt; ++k) dP[k] *= (a + b + n + 1) / 2.0;
+    }
+    return dP;
+}
+
+// ----------------------------------------------------------------------------------------
+// Dubiner basis on the (-1,-1),(1,-1),(-1,1) triangle (basis/bases_2D/basis_dubiner_2D.cpp).
+static void dof_pq(unsigned dof, unsigned& p, unsigned& q) {  // :18-23
+    unsigned t = (unsigned)std::ceil((-3. + std::sqrt(1. + 8. * (dof + 1))) / 2.);
+    unsigned lower = (t + 1) * t / 2;
+    p = dof - lower;
+    q = t - p;
+}
+
+static std::array<double, 2> singular_dz1(unsigned q) {  // :177-193
+    if (q == 0) return {3., 1.};
+    auto t = singular_dz1(q - 1);
+    return {t[0] + q + 2, t[0] + t[1]};
+}
+
+static std::array<double, 2> singular_dz2(unsigned q) {  // :195-211
+    if (q == 0) return {0.5, 0};
+    auto t = singular_dz2(q - 1);
+    return {t[0] + 0.5 * (q + 1), t[0] + t[1]};
+}
+
+// phi: ndof x npt; dphi[0], dphi[1]: d/dz1, d/dz2 (GetPhi :4-29, GetDPhi :31-63)
+static void dubiner(unsigned P, const std::vector<std::array<double, 2>>& pts, Mat* phi, Mat* dphi1, Mat* dphi2) {
+    unsigned ndof = (P + 1) * (P + 2) / 2;
+    size_t npt = pts.size();
+    std::vector<double> n1(npt), n2(npt);
+    for (size_t k = 0; k < npt; ++k) {
+        n1[k] = 2 * (1 + pts[k][0]) / (1 - pts[k][1]) - 1;
+        n2[k] = pts[k][1];
+    }
+    if (phi) *phi = Mat(ndof, (int)npt);
+    if (dphi1) *dphi1 = Mat(ndof, (int)npt);
+    if (dphi2) *dphi2 = Mat(ndof, (int)npt);
+    for (unsigned dof = 0; dof < ndof; ++dof) {
+        unsigned p, q;
+        dof_pq(dof, p, q);
+        std::vector<double> psi_p = jacobi(p, 0, 0, n1);
+        std::vector<double> psi_pq = jacobi(q, 2 * p + 1, 0, n2);
+        std::vector<double> dpsi_p = jacobi_derivative(p, 0, 0, n1);
+        std::vector<double> dpsi_pq = jacobi_derivative(q, 2 * p + 1, 0, n2);
+        for (size_t k = 0; k < npt; ++k) {
+            double s = (1 - n2[k]) / 2;
+            bool sing = std::isnan(n1[k]);
+            if (phi) {  // ComputePhi :85-111
+                double v = psi_p[k] * std::pow(s, (double)(int)p) * psi_pq[k];
+                if (sing) v = (p == 0) ? (double)(q + 1) : 0.0;
+                (*phi)(dof, (int)k) = v;
+            }
+            if (dphi1) {  // ComputeDPhiDZ1 :113-139
+                double v = (2.0 / (1.0 - n2[k])) * dpsi_p[k] * std::pow((1.0 - n2[k]) / 2.0, (double)(int)p) *
+                           psi_pq[k];
+                if (sing) v = (p == 1) ? singular_dz1(q)[1] : 0.0;
+                (*dphi1)(dof, (int)k) = v;
+            }
+            if (dphi2) {  // ComputeDPhiDZ2 :141-175
+                double s2 = (1.0 - n2[k]) / 2.0;
+                double v = ((1 + n1[k]) / (1.0 - n2[k])) * dpsi_p[k] * std::pow(s2, (double)(int)p) * psi_pq[k] +
+                           psi_p[k] * (std::pow(s2, (double)(int)p) * dpsi_pq[k] -
+                                       (p / 2.0) * std::pow(s2, (double)(int)(p - 1)) * psi_pq[k]);
+                if (sing) {
+                    if (p == 0)
+                        v = 3 * singular_dz2(q)[1];
+                    else if (p == 1)
+                        v = singular_dz2(q + 1)[1];
+                    else
+                        v = 0.0;
+                }
+                (*dphi2)(dof, (int)k) = v;
+            }
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------------
+// Quadrature (integration_dunavant_2D.cpp:4-76, integration_gausslegendre_1D.cpp:4-30).
+struct Rule2D {
+    std::vector<double> w;
+    std::vector<std::array<double, 2>> z;
+};
+struct Rule1D {
+    std::vector<double> w;
+    std::vector<double> z;
+};
+
+static Rule2D dunavant(unsigned p) {
+    if (p > 20) {
+        fprintf(stderr, "oracle: illegal Dunavant order %u\n", p);
+        std::abort();
+    }
+    unsigned r = p == 0 ? 1 : p;
+    const int first = rkdg_quad::kDunavantIndex[r - 1][0], cnt = rkdg_quad::kDunavantIndex[r - 1][1];
+    Rule2D R;
+    for (int i = first; i < first + cnt; ++i) {
+        const auto& o = rkdg_quad::kDunavantOrbits[i];
+        double l[3] = {o.l0, o.l1, o.l2};
+        auto push = [&](int a, int b) {
+            R.w.push_back(2 * o.w);
+            R.z.push_back({2 * l[a] - 1, 2 * l[b] - 1});
+        };
+        if (o.orbit == 1) {
+            push(0, 1);
+        } else if (o.orbit == 3) {
+            push(0, 1), push(1, 2), push(2, 0);
+        } else {
+            push(0, 1), push(1, 2), push(2, 0), push(1, 0), push(2, 1), push(0, 2);
+        }
+    }
+    return R;
+}
+
+static Rule1D gauss_legendre(unsigned p) {
+    unsigned n = (unsigned)std::ceil(((double)p + 1) / 2);
+    Rule1D R;
+    int first = rkdg_quad::kGaussLegendreIndex[n - 1];
+    for (unsigned i = 0; i < n; ++i) {
+        R.w.push_back(rkdg_quad::kGaussLegendre[first + i].w);
+        R.z.push_back(rkdg_quad::kGaussLegendre[first + i].z);
+    }
+    return R;
+}
+
+// ----------------------------------------------------------------------------------------
+// Master triangle (master/elements_2D/master_triangle.tpp:5-122).
+struct Master {
+    unsigned p = 0, ndof = 0, ngp = 0;
+    Rule2D rule;
+    Mat phi, dphi[2];          // ndof x ngp
+    Mat int_phi_fact;          // ngp x ndof
+    Mat int_dphi_fact[2];      // ngp x ndof
+    std::vector<double> minv;  // diagonal
+    Mat T_bl, T_lb;            // ndof x 3, 3 x ndof
+    double dchi_gp[2][3];      // constant per gp (:35-59)
+};
+
+static Master make_master(unsigned p) {
+    Master m;
+    m.p = p;
+    m.rule = dunavant(2 * p);
+    m.ndof = (p + 1) * (p + 2) / 2;
+    m.ngp = (unsigned)m.rule.w.size();
+    dubiner(p, m.rule.z, &m.phi, &m.dphi[0], &m.dphi[1]);
+    m.int_phi_fact = Mat(m.ngp, m.ndof);
+    for (unsigned d = 0; d < m.ndof; ++d)
+        for (unsigned g = 0; g < m.ngp; ++g) m.int_phi_fact(g, d) = m.phi(d, g) * m.rule.w[g];
+    for (int dir = 0; dir < 2; ++dir) {
+        m.int_dphi_fact[dir] = Mat(m.ngp, m.ndof);
+        for (unsigned d = 0; d < m.ndof; ++d)
+            for (unsigned g = 0; g < m.ngp; ++g) m.int_dphi_fact[dir](g, d) = m.dphi[dir](d, g) * m.rule.w[g];
+    }
+    m.minv.resize(m.ndof);
+    for (unsigned d = 0; d < m.ndof; ++d) {  // GetMinv basis_dubiner_2D.cpp:65-83
+        unsigned pp, qq;
+        dof_pq(d, pp, qq);
+        m.minv[d] = ((2 * pp + 1) * (pp + qq + 1) / 2.0);
+    }
+    m.T_bl = Mat(m.ndof, 3);  // GetBasisLinearT :213-234
+    m.T_bl(0, 0) = 1.0, m.T_bl(1, 0) = -1.0, m.T_bl(2, 0) = -1.0;
+    m.T_bl(0, 1) = 1.0, m.T_bl(1, 1) = -1.0, m.T_bl(2, 1) = 1.0;
+    m.T_bl(0, 2) = 1.0, m.T_bl(1, 2) = 2.0, m.T_bl(2, 2) = 0.0;
+    m.T_lb = Mat(3, m.ndof);  // GetLinearBasisT :236-257
+    m.T_lb(0, 0) = 1.0 / 3.0, m.T_lb(1, 0) = 1.0 / 3.0, m.T_lb(2, 0) = 1.0 / 3.0;
+    m.T_lb(0, 1) = -1.0 / 6.0, m.T_lb(1, 1) = -1.0 / 6.0, m.T_lb(2, 1) = 1.0 / 3.0;
+    m.T_lb(0, 2) = -1.0 / 2.0, m.T_lb(1, 2) = 1.0 / 2.0, m.T_lb(2, 2) = 0.0;
+    return m;
+}
+
+// Edge k of the master triangle for GL points z (BoundaryToMasterCoordinates :89-122).
+static std::vector<std::array<double, 2>> boundary_to_master(unsigned bound_id, const std::vector<double>& z) {
+    std::vector<std::array<double, 2>> out(z.size());
+    for (size_t g = 0; g < z.size(); ++g) {
+        if (bound_id == 0)
+            out[g] = {-z[g], z[g]};
+        else if (bound_id == 1)
+            out[g] = {-1.0, -z[g]};
+        else
+            out[g] = {z[g], -1.0};
+    }
+    return out;
+}
+
+// ----------------------------------------------------------------------------------------
+// Straight triangle shape (shape/shapes_2D/shape_straighttriangle.cpp).
+struct Shape {
+    double x[3], y[3], z[3];
+
+    void J(double M[2][2]) const {  // :95-108
+        M[0][0] = (x[1] - x[0]) / 2.0;
+        M[0][1] = (x[2] - x[0]) / 2.0;
+        M[1][0] = (y[1] - y[0]) / 2.0;
+        M[1][1] = (y[2] - y[0]) / 2.0;
+    }
+    double detJ() const {
+        double M[2][2];
+        J(M);
+        return M[0][0] * M[1][1] - M[1][0] * M[0][1];
+    }
+    void Jinv(double I[2][2]) const {  // :124-137, Eigen 2x2 inverse
+        double M[2][2];
+        J(M);
+        double det = M[0][0] * M[1][1] - M[1][0] * M[0][1];
+        double invdet = 1.0 / det;
+        I[0][0] = M[1][1] * invdet;
+        I[1][0] = -M[1][0] * invdet;
+        I[0][1] = -M[0][1] * invdet;
+        I[1][1] = M[0][0] * invdet;
+    }
+    double surfaceJ(unsigned b) const {  // :110-122
+        return std::sqrt(std::pow(x[(b + 2) % 3] - x[(b + 1) % 3], 2.0) +
+                         std::pow(y[(b + 2) % 3] - y[(b + 1) % 3], 2.0)) /
+               2.0;
+    }
+    std::array<double, 2> normal(unsigned b) const {  // :139-168
+        double det = detJ();
+        double cw = det / std::abs(det);
+        double length = std::hypot(x[(b + 2) % 3] - x[(b + 1) % 3], y[(b + 2) % 3] - y[(b + 1) % 3]);
+        return {cw * (y[(b + 2) % 3] - y[(b + 1) % 3]) / length, -cw * (x[(b + 2) % 3] - x[(b + 1) % 3]) / length};
+    }
+    double area() const {  // :23-30
+        return 0.5 * (x[0] * (y[1] - y[2]) + x[1] * (y[2] - y[0]) + x[2] * (y[0] - y[1]));
+    }
+    double radius() const {  // :32-40
+        double ret = 1.0 / (4 * area());
+        for (unsigned s = 0; s < 3; ++s) ret *= std::hypot(x[s] - x[(s + 1) % 3], y[s] - y[(s + 1) % 3]);
+        return ret;
+    }
+    std::array<double, 3> lengths() const {  // :52-61
+        std::array<double, 3> r;
+        for (unsigned s = 0; s < 3; ++s) r[s] = std::hypot(x[(s + 1) % 3] - x[(s + 2) % 3], y[(s + 1) % 3] - y[(s + 2) % 3]);
+        return r;
+    }
+    std::array<double, 2> baryctr() const {  // :63-76
+        return {(x[0] + x[1] + x[2]) / 3.0, (y[0] + y[1] + y[2]) / 3.0};
+    }
+    std::array<double, 2> midpt(unsigned k) const {  // :78-92
+        return {(x[(k + 1) % 3] + x[(k + 2) % 3]) / 2.0, (y[(k + 1) % 3] + y[(k + 2) % 3]) / 2.0};
+    }
+    // GetPsi (:170-182): 3 x npt
+    static Mat psi(const std::vector<std::array<double, 2>>& pts) {
+        Mat P(3, (int)pts.size());
+        for (size_t k = 0; k < pts.size(); ++k) {
+            P(0, (int)k) = -(pts[k][0] + pts[k][1]) / 2;
+            P(1, (int)k) = (1 + pts[k][0]) / 2;
+            P(2, (int)k) = (1 + pts[k][1]) / 2;
+        }
+        return P;
+    }
+    std::vector<std::array<double, 2>> local_to_global(const std::vector<std::array<double, 2>>& pts) const {
+        Mat P = psi(pts);  // :229-247
+        std::vector<std::array<double, 2>> g(pts.size());
+        for (size_t k = 0; k < pts.size(); ++k) {
+            g[k][0] = x[0] * P(0, (int)k) + x[1] * P(1, (int)k) + x[2] * P(2, (int)k);
+            g[k][1] = y[0] * P(0, (int)k) + y[1] * P(1, (int)k) + y[2] * P(2, (int)k);
+        }
+        return g;
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// Problem closed forms (examples/swe_manufactured_solution/*.hpp, examples/swe_parabolic_bowl/*.hpp).
+struct Problem {
+    int kind = RKDG_PROBLEM_NONE;
+    double g = 9.81;
+    double param[4] = {0, 0, 0, 0};
+
+    std::array<double, 3> ic(double t, double x, double y) const {
+        if (kind == RKDG_PROBLEM_MANUFACTURED) {  // manufactured_swe_initial_condition_functions.hpp
+            return {std::exp(std::sin(3 * x) * std::sin(3 * y) - std::sin(3 * t)), std::cos(x - 4 * t),
+                    std::sin(y + 4 * t)};
+        } else if (kind == RKDG_PROBLEM_PARABOLIC_BOWL) {  // parabolic_bowl_initial_condition_functions.hpp:7-28
+            const double alpha = 1.e-7, X = 1, Y = -0.41884;
+            double r2 = x * x + y * y;
+            double bathymetry = -alpha * r2;
+            double H = 1 / (X + Y) + alpha * (Y * Y - X * X) / ((X + Y) * (X + Y)) * r2;
+            return {H - bathymetry, 0, 0};
+        } else if (kind == RKDG_PROBLEM_DAM_BREAK) {  // step of the free surface at x0
+            return {x < param[0] ? param[1] : param[2], 0.0, 0.0};
+        }
+        return {0, 0, 0};
+    }
+    std::array<double, 3> true_q(double t, double x, double y) const { return ic(t, x, y); }
+    std::array<double, 3> source(double t, double x, double y) const {
+        if (kind != RKDG_PROBLEM_MANUFACTURED) return {0, 0, 0};
+        // manufactured_swe_source_functions.hpp
+        double sze = -3 * std::exp(-std::sin(3 * t) + std::sin(3 * x) * std::sin(3 * y)) * std::cos(3 * t) +
+                     std::cos(4 * t + y) + std::sin(4 * t - x);
+        double E = std::exp(-std::sin(3 * t) + std::sin(3 * x) * std::sin(3 * y));
+        double sqx =
+            (std::cos(4 * t - x) * std::cos(4 * t + y)) / (2 + E) - 4 * std::sin(4 * t - x) +
+            (2 * std::cos(4 * t - x) * std::sin(4 * t - x)) / (2 + E) +
+            3. * E * (2 + E) * std::cos(3 * x) * std::sin(3 * y) -
+            (3 * E * std::pow(std::cos(4 * t - x), 2) * std::cos(3 * x) * std::sin(3 * y)) / std::pow(2 + E, 2) -
+            (3 * E * std::cos(4 * t - x) * std::cos(3 * y) * std::sin(3 * x) * std::sin(4 * t + y)) /
+                std::pow(2 + E, 2);
+        double sqy = 4 * std::cos(4 * t + y) + 3. * E * (2 + E) * std::cos(3 * y) * std::sin(3 * x) +
+                     (2 * std::cos(4 * t + y) * std::sin(4 * t + y)) / (2 + E) +
+                     (std::sin(4 * t - x) * std::sin(4 * t + y)) / (2 + E) -
+                     (3 * E * std::cos(4 * t - x) * std::cos(3 * x) * std::sin(3 * y) * std::sin(4 * t + y)) /
+                         std::pow(2 + E, 2) -
+                     (3 * E * std::cos(3 * y) * std::sin(3 * x) * std::pow(std::sin(4 * t + y), 2)) /
+                         std::pow(2 + E, 2);
+        return {sze, sqx, sqy};
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// LLF flux (numerical_fluxes/rkdg_swe_LLF_flux.hpp:7-60).  aux = {bath, h, sp}.
+static void LLF_flux(double gravity, const double* q_in, const double* q_ex, const double* aux, const double* n,
+                     double* F_hat) {
+    double bath = aux[0];
+    double sp = aux[2];
+    double h_in = q_in[0] + bath;
+    double u_in = q_in[1] / h_in;
+    double v_in = q_in[2] / h_in;
+    double h_ex = q_ex[0] + bath;
+    double u_ex = q_ex[1] / h_ex;
+    double v_ex = q_ex[2] / h_ex;
+    double un_in = u_in * n[0] + v_in * n[1];
+    double un_ex = u_ex * n[0] + v_ex * n[1];
+    double sp_correction = std::pow(n[0] * sp, 2) + std::pow(n[1], 2);
+    double max_eigenvalue = std::max(std::abs(un_in) + std::sqrt(gravity * h_in * sp_correction),
+                                     std::abs(un_ex) + std::sqrt(gravity * h_ex * sp_correction));
+    double nx = n[0], ny = n[1];
+    double uuh_in = u_in * q_in[1];
+    double vvh_in = v_in * q_in[2];
+    double uvh_in = u_in * q_in[2];
+    double pe_in = gravity * (std::pow(q_in[0], 2) / 2 + q_in[0] * bath);
+    double Fn_in[3] = {sp * q_in[1] * nx + q_in[2] * ny, sp * (uuh_in + pe_in) * nx + uvh_in * ny,
+                       sp * uvh_in * nx + (vvh_in + pe_in) * ny};
+    double uuh_ex = u_ex * q_ex[1];
+    double vvh_ex = v_ex * q_ex[2];
+    double uvh_ex = u_ex * q_ex[2];
+    double pe_ex = gravity * (std::pow(q_ex[0], 2) / 2 + q_ex[0] * bath);
+    double Fn_ex[3] = {q_ex[1] * nx + q_ex[2] * ny, (uuh_ex + pe_ex) * nx + uvh_ex * ny,
+                       uvh_ex * nx + (vvh_ex + pe_ex) * ny};
+    for (int v = 0; v < 3; ++v) F_hat[v] = 0.5 * (Fn_in[v] + Fn_ex[v] + max_eigenvalue * (q_in[v] - q_ex[v]));
+}
+
+// Land BC state (boundary_conditions/rkdg_swe_bc_land.hpp:68-86).
+static void land_flux(double g, const double* n, const double* q_in, const double* aux, double* F) {
+    double n_x = n[0], n_y = n[1], t_x = -n_y, t_y = n_x;
+    double qn_ex = -(q_in[1] * n_x + q_in[2] * n_y);
+    double qt_ex = q_in[1] * t_x + q_in[2] * t_y;
+    double q_ex[3] = {q_in[0], qn_ex * n_x + qt_ex * t_x, qn_ex * n_y + qt_ex * t_y};
+    LLF_flux(g, q_in, q_ex, aux, n, F);
+}
+
+// ----------------------------------------------------------------------------------------
+// SSP-RK stepper (simulation/stepper/explicit_ssp_rk_stepper.{hpp,cpp}).
+struct Stepper {
+    unsigned order = 2, nstages = 2;
+    double dt = 0;
+    unsigned step = 0, stage = 0, timestamp = 0;
+    double t = 0, ramp_duration = 0, ramp = 1;
+    std::vector<std::vector<double>> ark, brk, crk;
+    std::vector<double> drk;
+
+    bool init(unsigned ns, unsigned ord, double dt_, double ramp_dur) {
+        nstages = ns, order = ord, dt = dt_, ramp_duration = ramp_dur;
+        ramp = (std::abs(ramp_duration) < std::numeric_limits<double>::epsilon() * 100) ? 1. : 0.;
+        ark.assign(ns, {}), brk.assign(ns, {}), crk.assign(ns, {});
+        drk.assign(ns, 0.0);
+        for (unsigned s = 1; s <= ns; ++s) {
+            ark[s - 1].assign(s, 0.0), brk[s - 1].assign(s, 0.0), crk[s - 1].assign(s, 0.0);
+        }
+        // InitializeCoefficients (.cpp:16-257): Shu-Osher tables.
+        auto& a = ark;
+        auto& b = brk;
+        if (ns == 1 && ord == 1) {
+            a[0][0] = 1., b[0][0] = 1.;
+        } else if (ns == 2 && ord == 2) {
+            a[0][0] = 1, a[1][0] = 0.5, a[1][1] = 0.5;
+            b[0][0] = 1., b[1][1] = 0.5;
+        } else if (ns == 3 && ord == 3) {
+            a[0][0] = 1., a[1][0] = 3. / 4., a[1][1] = 1. / 4., a[2][0] = 1. / 3., a[2][2] = 2. / 3.;
+            b[0][0] = 1., b[1][1] = 1. / 4., b[2][2] = 2. / 3.;
+        } else if (ns == 4 && ord == 3) {
+            a[0][0] = 1., a[1][1] = 1., a[2][0] = 2. / 3., a[2][2] = 1. / 3., a[3][3] = 1.;
+            b[0][0] = 1. / 2., b[1][1] = 1. / 2., b[2][2] = 1. / 6., b[3][3] = 1. / 2.;
+        } else {
+            return false;  // other schemes: not restated here (the engine's own table covers them)
+        }
+        // time-dependent parameters (.cpp:259-276)
+        for (unsigned i = 0; i < ns; ++i)
+            for (unsigned k = 0; k < i; ++k) {
+                double casum = 0.;
+                for (unsigned l = k + 1; l < i; ++l) casum += crk[l][k] * ark[i][l];
+                crk[i][k] = brk[i][k] + casum;
+            }
+        for (unsigned k = 1; k < ns; ++k)
+            for (unsigned l = 0; l < k; ++l) drk[k] = drk[k] + crk[k - 1][l];
+        return true;
+    }
+    double time_at_stage() const { return t + dt * drk[stage]; }
+    void advance() {  // operator++ (.hpp:49-65)
+        ++stage, ++timestamp;
+        stage = stage % nstages;
+        if (stage == 0) {
+            t += dt;
+            ++step;
+        }
+        if (!(std::abs(ramp_duration) < std::numeric_limits<double>::epsilon() * 100))
+            ramp = std::tanh(2 * (time_at_stage() / 86400) / ramp_duration);
+    }
+};
+
+// ----------------------------------------------------------------------------------------
+// Mesh objects.
+struct Element {
+    unsigned id;
+    unsigned node_ID[3];
+    unsigned neighbor_ID[3];
+    unsigned char btype[3];  // 0..3 BC codes, 128 internal
+    Shape shape;
+    Mat dphi_gp[2];          // ndof x ngp (element, global derivative)
+    std::vector<double> int_fact;
+    Mat int_phi_fact;        // ngp x ndof
+    Mat int_dphi_fact[2];    // ngp x ndof
+    std::vector<double> minv;
+    std::vector<std::array<double, 2>> gp_global;
+    // data
+    std::vector<Mat> q, rhs, solution;  // per stage, 3 x ndof
+    Mat bath;                           // 1 x ndof
+    Mat aux_at_gp;                      // 3 x ngp (bath, h, sp)
+    Mat db_at_gp;                       // 2 x ngp
+    Mat q_at_gp;                        // 3 x ngp
+    bool manning = false;
+    double g_manning_n_sq = 0;
+    // wet-dry state (problem_data_structure/swe_data_wet_dry.hpp)
+    bool wet = true, went_completely_dry = false;
+    double bath_at_vrtx[3], bath_min = 0;
+    // slope limit state (swe_data_slope_limit.hpp)
+    std::array<double, 3> lengths;
+    double radius = 0, perimeter = 0, I = 0;
+    bool troubled = false;
+    bool limited[3] = {false, false, false};
+    std::array<double, 2> baryctr_coord, midpts_coord[3], baryctr_coord_neigh[3], median[3], alpha[3];
+    double r_sq[3];
+    Mat sl_q_lin, sl_q_at_vrtx, sl_q_at_midpts;  // 3 x 3
+    double q_at_baryctr[3];
+    double sl_bath_at_vrtx[3], bath_at_baryctr;
+    bool wet_neigh[3];
+    double q_at_baryctr_neigh[3][3];
+};
+
+struct EdgeSide {
+    unsigned elt, bound;
+    Mat phi;                  // ndof x ngp
+    std::vector<double> int_fact;
+    Mat int_phi_fact;         // ngp x ndof
+    std::vector<std::array<double, 2>> normal;  // per gp
+    Mat aux_at_gp;            // 3 x ngp
+    Mat q_at_gp;              // 3 x ngp
+    Mat F_hat;                // 3 x ngp
+    std::vector<std::array<double, 2>> gp_global;
+    Mat psi_bound;            // 2 x ngp
+    unsigned node_ID[2];
+};
+
+struct Interface {
+    EdgeSide in, ex;
+};
+
+struct Boundary {
+    unsigned char type;
+    EdgeSide s;
+    std::vector<std::vector<double>> amplitude_gp, phase_gp;  // [con][gp]
+};
+
+// ----------------------------------------------------------------------------------------
+struct Sim {
+    Master master;
+    Stepper stepper;
+    Problem prob;
+    rkdg_params prm;
+    std::vector<Element> elts;
+    std::vector<Interface> intfaces;
+    std::vector<Boundary> bounds;  // grouped Land, Tide, Flow, Function; each by (elt, face)
+    std::vector<double> frequency, forcing_fact, equilib_arg;
+    double h_o = 0.1, h_o_threshold = 1.0e-6, M = 1.0e-8, nu = 1.5;
+    int nthreads = 1;
+    std::vector<std::vector<unsigned>> intface_colours;  // for the OpenMP path
+    int nan_elt = -1, nan_step = -1, nan_stage = -1;
+    unsigned ndof() const { return master.ndof; }
+};
+
+static void make_side(const Sim& S, EdgeSide& s, const Element& e, unsigned bound, const Rule1D& rule, double sJ,
+                      const std::array<double, 2>& normal, bool reversed_normal) {
+    s.elt = e.id;
+    s.bound = bound;
+    unsigned ngp = (unsigned)rule.w.size();
+    auto zm = boundary_to_master(bound, rule.z);
+    s.gp_global = e.shape.local_to_global(zm);
+    dubiner(S.master.p, zm, &s.phi, nullptr, nullptr);
+    s.int_fact.resize(ngp);
+    for (unsigned g = 0; g < ngp; ++g) s.int_fact[g] = rule.w[g] * sJ;
+    s.int_phi_fact = Mat(ngp, S.master.ndof);
+    for (unsigned d = 0; d < S.master.ndof; ++d)
+        for (unsigned g = 0; g < ngp; ++g) s.int_phi_fact(g, d) = s.phi(d, g) * (rule.w[g] * sJ);
+    s.normal.resize(ngp);
+    for (unsigned g = 0; g < ngp; ++g) {
+        if (!reversed_normal)
+            s.normal[g] = normal;
+        else
+            s.normal[ngp - g - 1] = {-normal[0], -normal[1]};
+    }
+    s.psi_bound = Mat(2, ngp);  // GetBoundaryPsi :186-199
+    for (unsigned g = 0; g < ngp; ++g) {
+        s.psi_bound(0, g) = (1 - rule.z[g]) / 2;
+        s.psi_bound(1, g) = (1 + rule.z[g]) / 2;
+    }
+    s.node_ID[0] = e.node_ID[(bound + 1) % 3];  // GetBoundaryNodeID :14-21
+    s.node_ID[1] = e.node_ID[(bound + 2) % 3];
+    s.aux_at_gp = Mat(3, ngp);
+    s.q_at_gp = Mat(3, ngp);
+    s.F_hat = Mat(3, ngp);
+}
+
+static const char* g_err = "";
+
+static Sim* create(const rkdg_mesh_desc* md, const rkdg_params* prm) {
+    Sim* S = new Sim();
+    S->prm = *prm;
+    S->prob.kind = prm->problem;
+    S->prob.g = prm->g;
+    for (int i = 0; i < 4; ++i) S->prob.param[i] = prm->problem_param[i];
+    if (!S->stepper.init(prm->nstages, prm->order, prm->dt, prm->ramp_duration)) {
+        g_err = "unsupported RK scheme";
+        delete S;
+        return nullptr;
+    }
+    if (prm->wetting_drying) {  // swe_pre_init_prob_params.hpp:33-37
+        S->h_o = prm->h_o;
+        S->h_o_threshold = 1.0e6 * S->h_o * std::numeric_limits<double>::epsilon();
+    }
+    if (prm->slope_limiting) S->M = prm->sl_M, S->nu = prm->sl_nu;
+    S->master = make_master(prm->p);
+    const Master& m = S->master;
+    unsigned ne = md->n_elements, ndof = m.ndof, ngp = m.ngp;
+    for (int c = 0; c < md->n_constituents; ++c) {
+        S->frequency.push_back(md->frequency[c]);
+        S->forcing_fact.push_back(md->forcing_fact[c]);
+        S->equilib_arg.push_back(md->equilib_arg[c]);
+    }
+
+    // boundary typing by node pair (mesh_metadata.cpp:31-85 with get_ibtype)
+    std::map<std::pair<unsigned, unsigned>, unsigned char> btype_of;
+    for (int i = 0; i < md->n_bound_edges; ++i) {
+        unsigned a = md->bound_edge_nodes[2 * i], b = md->bound_edge_nodes[2 * i + 1];
+        btype_of[{std::min(a, b), std::max(a, b)}] = md->bound_edge_type[i];
+    }
+    std::unordered_map<uint64_t, std::array<unsigned, 4>> dict;  // key -> eltA, faceA, eltB, faceB
+    S->elts.resize(ne);
+    for (unsigned e = 0; e < ne; ++e) {
+        Element& E = S->elts[e];
+        E.id = e;
+        for (int k = 0; k < 3; ++k) E.node_ID[k] = md->elt_nodes[3 * e + k];
+        for (int k = 0; k < 3; ++k) {
+            unsigned a = E.node_ID[(k + 1) % 3], b = E.node_ID[(k + 2) % 3];
+            uint64_t key = (uint64_t)std::min(a, b) << 32 | std::max(a, b);
+            auto it = dict.find(key);
+            if (it != dict.end()) {
+                it->second[2] = e, it->second[3] = k;
+            } else {
+                dict[key] = {e, (unsigned)k, DEFAULT_ID, 0};
+            }
+        }
+    }
+    for (auto& kv : dict) {
+        auto& v = kv.second;
+        if (v[2] != DEFAULT_ID) {
+            S->elts[v[0]].neighbor_ID[v[1]] = v[2], S->elts[v[0]].btype[v[1]] = 128;
+            S->elts[v[2]].neighbor_ID[v[3]] = v[0], S->elts[v[2]].btype[v[3]] = 128;
+        } else {
+            unsigned a = (unsigned)(kv.first >> 32), b = (unsigned)(kv.first & 0xffffffffu);
+            auto it = btype_of.find({a, b});
+            if (it == btype_of.end()) {
+                g_err = "boundary edge without a boundary type";
+                delete S;
+                return nullptr;
+            }
+            S->elts[v[0]].neighbor_ID[v[1]] = DEFAULT_ID;
+            S->elts[v[0]].btype[v[1]] = it->second;
+        }
+    }
+
+    // elements (Element::Initialize, element.hpp:166-272)
+    Mat psi_gp = Shape::psi(m.rule.z);
+    for (unsigned e = 0; e < ne; ++e) {
+        Element& E = S->elts[e];
+        for (int k = 0; k < 3; ++k) {
+            unsigned nd = E.node_ID[k];
+            E.shape.x[k] = md->node_xyz[3 * nd], E.shape.y[k] = md->node_xyz[3 * nd + 1];
+            E.shape.z[k] = md->node_xyz[3 * nd + 2];
+        }
+        if (E.shape.detJ() < 0) {  // CCW fix swaps coordinates only (shape_straighttriangle.cpp:9-11)
+            std::swap(E.shape.x[0], E.shape.x[2]);
+            std::swap(E.shape.y[0], E.shape.y[2]);
+            std::swap(E.shape.z[0], E.shape.z[2]);
+        }
+        E.gp_global = E.shape.local_to_global(m.rule.z);
+        double det = E.shape.detJ();
+        double Ji[2][2];
+        E.shape.Jinv(Ji);
+        for (int dir = 0; dir < 2; ++dir) {
+            E.dphi_gp[dir] = Mat(ndof, ngp);
+            for (unsigned g = 0; g < ngp; ++g)
+                for (unsigned d = 0; d < ndof; ++d) {
+                    double dphi = 0.0;
+                    for (int z = 0; z < 2; ++z) dphi += m.dphi[z](d, g) * Ji[z][dir];
+                    E.dphi_gp[dir](d, g) = dphi;
+                }
+        }
+        E.int_fact.resize(ngp);
+        for (unsigned g = 0; g < ngp; ++g) E.int_fact[g] = m.rule.w[g] * std::abs(det);
+        E.int_phi_fact = Mat(ngp, ndof);
+        for (unsigned g = 0; g < ngp; ++g)
+            for (unsigned d = 0; d < ndof; ++d) E.int_phi_fact(g, d) = m.int_phi_fact(g, d) * std::abs(det);
+        for (int dir = 0; dir < 2; ++dir) {
+            E.int_dphi_fact[dir] = Mat(ngp, ndof);
+            for (unsigned d = 0; d < ndof; ++d)
+                for (unsigned g = 0; g < ngp; ++g) {
+                    double v = 0;
+                    for (int z = 0; z < 2; ++z) v += m.int_dphi_fact[z](g, d) * Ji[z][dir];
+                    v *= std::abs(det);
+                    E.int_dphi_fact[dir](g, d) = v;
+                }
+        }
+        E.minv.resize(ndof);
+        for (unsigned d = 0; d < ndof; ++d) E.minv[d] = m.minv[d] / std::abs(det);
+        unsigned ns = S->stepper.nstages;
+        E.q.assign(ns + 1, Mat(3, ndof));
+        E.rhs.assign(ns + 1, Mat(3, ndof));
+        E.solution.assign(ns + 1, Mat(3, ndof));
+        E.aux_at_gp = Mat(3, ngp);
+        E.db_at_gp = Mat(2, ngp);
+        E.q_at_gp = Mat(3, ngp);
+        E.sl_q_lin = Mat(3, 3), E.sl_q_at_vrtx = Mat(3, 3), E.sl_q_at_midpts = Mat(3, 3);
+    }
+
+    // interfaces: raw boundaries keyed (ID, neighbour); "in" = lower ID (swe_pre_create_intface.hpp:14-31)
+    Rule1D erule = gauss_legendre(2 * m.p + 1);
+    for (unsigned e = 0; e < ne; ++e) {
+        Element& E = S->elts[e];
+        // faces of E with a higher-ID neighbour, ordered by neighbour ID (map key order)
+        std::vector<std::pair<unsigned, unsigned>> nb;
+        for (unsigned k = 0; k < 3; ++k)
+            if (E.btype[k] == 128 && E.neighbor_ID[k] > e) nb.push_back({E.neighbor_ID[k], k});
+        std::sort(nb.begin(), nb.end());
+        for (auto& pr : nb) {
+            Element& X = S->elts[pr.first];
+            unsigned bx = 0;
+            for (unsigned k = 0; k < 3; ++k)
+                if (X.btype[k] == 128 && X.neighbor_ID[k] == e) bx = k;
+            Interface I;
+            double sJ = E.shape.surfaceJ(pr.second);
+            auto nrm = E.shape.normal(pr.second);
+            make_side(*S, I.in, E, pr.second, erule, sJ, nrm, false);
+            make_side(*S, I.ex, X, bx, erule, sJ, nrm, true);
+            S->intfaces.push_back(std::move(I));
+        }
+    }
+    // boundaries grouped by type (Land, Tide, Flow, Function), each keyed (ID, face)
+    for (unsigned char ty = 0; ty < 4; ++ty)
+        for (unsigned e = 0; e < ne; ++e)
+            for (unsigned k = 0; k < 3; ++k) {
+                Element& E = S->elts[e];
+                if (E.btype[k] != ty) continue;
+                Boundary B;
+                B.type = ty;
+                make_side(*S, B.s, E, k, erule, E.shape.surfaceJ(k), E.shape.normal(k), false);
+                if (ty == RKDG_BC_TIDE || ty == RKDG_BC_FLOW) {  // Tide::Initialize (rkdg_swe_bc_tide.hpp:54-68)
+                    unsigned ncon = S->frequency.size();
+                    B.amplitude_gp.assign(ncon, std::vector<double>(B.s.int_fact.size()));
+                    B.phase_gp.assign(ncon, std::vector<double>(B.s.int_fact.size()));
+                    for (unsigned c = 0; c < ncon; ++c) {
+                        double a0 = md->node_amplitude[(size_t)B.s.node_ID[0] * ncon + c];
+                        double a1 = md->node_amplitude[(size_t)B.s.node_ID[1] * ncon + c];
+                        double p0 = md->node_phase[(size_t)B.s.node_ID[0] * ncon + c];
+                        double p1 = md->node_phase[(size_t)B.s.node_ID[1] * ncon + c];
+                        for (size_t g = 0; g < B.s.int_fact.size(); ++g) {
+                            B.amplitude_gp[c][g] = a0 * B.s.psi_bound(0, g) + a1 * B.s.psi_bound(1, g);
+                            B.phase_gp[c][g] = p0 * B.s.psi_bound(0, g) + p1 * B.s.psi_bound(1, g);
+                        }
+                    }
+                }
+                S->bounds.push_back(std::move(B));
+            }
+
+    // initialize_data_serial (swe_pre_init_data.hpp:8-327)
+    for (unsigned e = 0; e < ne; ++e) {
+        Element& E = S->elts[e];
+        Mat bz(1, 3);
+        for (int k = 0; k < 3; ++k) bz(0, k) = E.shape.z[k];
+        Mat IPF(ngp, ndof);
+        IPF = E.int_phi_fact;
+        Mat Minv(ndof, ndof);
+        for (unsigned d = 0; d < ndof; ++d) Minv(d, d) = E.minv[d];
+        E.bath = mul(mul(mul(bz, psi_gp), IPF), Minv);  // L2ProjectionNode (element.hpp:317-321)
+        Mat bgp = mul(E.bath, m.phi);
+        for (unsigned g = 0; g < ngp; ++g) E.aux_at_gp(0, g) = bgp(0, g);
+        for (int dir = 0; dir < 2; ++dir) {
+            Mat db = mul(E.bath, E.dphi_gp[dir]);
+            for (unsigned g = 0; g < ngp; ++g) E.db_at_gp(dir, g) = db(0, g);
+        }
+        for (unsigned g = 0; g < ngp; ++g) E.aux_at_gp(2, g) = 1.0;  // no spherical projection
+        if (prm->ic_constant) {
+            Mat un(3, 3);
+            for (int k = 0; k < 3; ++k) un(0, k) = prm->ze0, un(1, k) = prm->qx0, un(2, k) = prm->qy0;
+            E.q[0] = mul(mul(mul(un, psi_gp), IPF), Minv);
+        } else {
+            Mat f(3, ngp);
+            for (unsigned g = 0; g < ngp; ++g) {
+                auto v = S->prob.ic(0, E.gp_global[g][0], E.gp_global[g][1]);
+                f(0, g) = v[0], f(1, g) = v[1], f(2, g) = v[2];
+            }
+            E.q[0] = mul(mul(f, IPF), Minv);  // L2ProjectionF (element.hpp:307-313)
+        }
+    }
+    auto edge_bath = [&](EdgeSide& s) {
+        Mat b = mul(S->elts[s.elt].bath, s.phi);
+        for (size_t g = 0; g < s.int_fact.size(); ++g) s.aux_at_gp(0, (int)g) = b(0, (int)g), s.aux_at_gp(2, (int)g) = 1.0;
+    };
+    for (auto& I : S->intfaces) edge_bath(I.in), edge_bath(I.ex);
+    for (auto& B : S->bounds) edge_bath(B.s);
+    if (prm->friction == RKDG_FRICTION_MANNING) {  // :166-197
+        for (auto& E : S->elts) {
+            double nn[3];
+            for (int k = 0; k < 3; ++k) nn[k] = md->node_manning_n[E.node_ID[k]];
+            E.manning = true;
+            E.g_manning_n_sq = prm->g * std::pow(((0.0 + nn[0]) + nn[1] + nn[2]) / 3, 2);
+        }
+    }
+    if (prm->wetting_drying) {  // :213-250
+        for (auto& E : S->elts) {
+            for (int v = 0; v < 3; ++v) E.bath_at_vrtx[v] = E.shape.z[v];
+            E.bath_min = std::min(std::min(E.bath_at_vrtx[0], E.bath_at_vrtx[1]), E.bath_at_vrtx[2]);
+            Mat ql = mul(E.q[0], m.T_bl);
+            Mat qv = ql;
+            double h[3];
+            for (int v = 0; v < 3; ++v) h[v] = qv(0, v) + E.bath_at_vrtx[v];
+            bool set_wet = true;
+            for (int v = 0; v < 3; ++v)
+                if (h[v] <= S->h_o + S->h_o_threshold) {
+                    qv(0, v) = S->h_o - E.bath_at_vrtx[v];
+                    set_wet = false;
+                }
+            if (set_wet) {
+                E.wet = true;
+            } else {
+                E.wet = false;
+                for (int v = 0; v < 3; ++v) qv(1, v) = 0.0, qv(2, v) = 0.0;
+                E.q[0] = mul(qv, m.T_lb);
+            }
+        }
+    }
+    if (prm->slope_limiting) {  // :252-326
+        for (auto& E : S->elts) {
+            for (int v = 0; v < 3; ++v) E.sl_bath_at_vrtx[v] = E.shape.z[v];
+            E.bath_at_baryctr = E.sl_bath_at_vrtx[0] * (1.0 / 3.0) + E.sl_bath_at_vrtx[1] * (1.0 / 3.0) +
+                                E.sl_bath_at_vrtx[2] * (1.0 / 3.0);
+            E.baryctr_coord = E.shape.baryctr();
+            for (int k = 0; k < 3; ++k) E.midpts_coord[k] = E.shape.midpt(k);
+            E.lengths = E.shape.lengths();
+            E.radius = E.shape.radius();
+            for (int b = 0; b < 3; ++b) {
+                double mx = E.midpts_coord[b][0] - E.baryctr_coord[0];
+                double my = E.midpts_coord[b][1] - E.baryctr_coord[1];
+                double nrm = std::sqrt(mx * mx + my * my);
+                E.median[b] = {mx / nrm, my / nrm};
+            }
+        }
+        for (auto& I : S->intfaces) {
+            S->elts[I.in.elt].baryctr_coord_neigh[I.in.bound] = S->elts[I.ex.elt].baryctr_coord;
+            S->elts[I.ex.elt].baryctr_coord_neigh[I.ex.bound] = S->elts[I.in.elt].baryctr_coord;
+        }
+        for (auto& B : S->bounds) {
+            Element& E = S->elts[B.s.elt];
+            E.baryctr_coord_neigh[B.s.bound] = {2.0 * E.midpts_coord[B.s.bound][0] - E.baryctr_coord[0],
+                                                2.0 * E.midpts_coord[B.s.bound][1] - E.baryctr_coord[1]};
+        }
+        for (auto& E : S->elts) {
+            for (int b = 0; b < 3; ++b) {
+                int e1 = b, e2 = (b + 1) % 3;
+                double A[2][2] = {{E.baryctr_coord_neigh[e1][0] - E.baryctr_coord[0],
+                                   E.baryctr_coord_neigh[e2][0] - E.baryctr_coord[0]},
+                                  {E.baryctr_coord_neigh[e1][1] - E.baryctr_coord[1],
+                                   E.baryctr_coord_neigh[e2][1] - E.baryctr_coord[1]}};
+                double bb[2] = {E.midpts_coord[b][0] - E.baryctr_coord[0], E.midpts_coord[b][1] - E.baryctr_coord[1]};
+                double det = A[0][0] * A[1][1] - A[1][0] * A[0][1];
+                double invdet = 1.0 / det;
+                double Ai[2][2] = {{A[1][1] * invdet, -A[0][1] * invdet}, {-A[1][0] * invdet, A[0][0] * invdet}};
+                E.alpha[b] = {Ai[0][0] * bb[0] + Ai[0][1] * bb[1], Ai[1][0] * bb[0] + Ai[1][1] * bb[1]};
+                E.r_sq[b] = bb[0] * bb[0] + bb[1] * bb[1];
+            }
+        }
+    }
+    // colour interfaces for the OpenMP baseline path (no two interfaces of a colour share an element)
+    {
+        std::vector<int> col(S->intfaces.size(), -1);
+        std::vector<std::vector<int>> used(ne);
+        for (size_t i = 0; i < S->intfaces.size(); ++i) {
+            unsigned a = S->intfaces[i].in.elt, b = S->intfaces[i].ex.elt;
+            int c = 0;
+            while (std::find(used[a].begin(), used[a].end(), c) != used[a].end() ||
+                   std::find(used[b].begin(), used[b].end(), c) != used[b].end())
+                ++c;
+            col[i] = c;
+            used[a].push_back(c), used[b].push_back(c);
+            if ((int)S->intface_colours.size() <= c) S->intface_colours.resize(c + 1);
+            S->intface_colours[c].push_back((unsigned)i);
+        }
+    }
+    return S;
+}
+
+// ----------------------------------------------------------------------------------------
+// Kernels (kernels_processor/*.hpp).
+
+static void volume_kernel(Sim& S, Element& E) {  // rkdg_swe_proc_volume.hpp:8-35
+    const Master& m = S.master;
+    unsigned st = S.stepper.stage, ngp = m.ngp, ndof = m.ndof;
+    Mat& rhs = E.rhs[st];
+    std::fill(rhs.a.begin(), rhs.a.end(), 0.0);
+    if (!E.wet) return;
+    E.q_at_gp = mul(E.q[st], m.phi);
+    Mat Fx(3, ngp), Fy(3, ngp);
+    for (unsigned g = 0; g < ngp; ++g) {
+        E.aux_at_gp(1, g) = E.q_at_gp(0, g) + E.aux_at_gp(0, g);
+        double ze = E.q_at_gp(0, g), qx = E.q_at_gp(1, g), qy = E.q_at_gp(2, g), h = E.aux_at_gp(1, g);
+        double bath = E.aux_at_gp(0, g);
+        double u = qx / h, v = qy / h;  // get_F (problem_flux/swe_flux.hpp:5-25)
+        double uuh = u * qx, vvh = v * qy, uvh = u * qy;
+        double pe = S.prm.g * (0.5 * (ze * ze) + ze * bath);
+        Fx(0, g) = qx, Fx(1, g) = uuh + pe, Fx(2, g) = uvh;
+        Fy(0, g) = qy, Fy(1, g) = uvh, Fy(2, g) = vvh + pe;
+        for (int k = 0; k < 3; ++k) Fx(k, g) = E.aux_at_gp(2, g) * Fx(k, g);
+    }
+    Mat A = mul(Fx, E.int_dphi_fact[0]);
+    Mat B = mul(Fy, E.int_dphi_fact[1]);
+    for (int k = 0; k < 3; ++k)
+        for (unsigned d = 0; d < ndof; ++d) rhs(k, d) = A(k, d) + B(k, d);
+}
+
+static void source_kernel(Sim& S, Element& E) {  // rkdg_swe_proc_source.hpp:8-18, swe_source.hpp:8-97
+    if (!E.wet) return;
+    const Master& m = S.master;
+    unsigned st = S.stepper.stage, ngp = m.ngp, ndof = m.ndof;
+    double t = S.stepper.time_at_stage();
+    Mat src(3, ngp);
+    for (unsigned g = 0; g < ngp; ++g) {
+        src(0, g) = 0.0;
+        src(1, g) = S.prm.g * (E.db_at_gp(0, g) * E.q_at_gp(0, g));
+        src(2, g) = S.prm.g * (E.db_at_gp(1, g) * E.q_at_gp(0, g));
+    }
+    if (S.prm.function_source) {
+        for (unsigned g = 0; g < ngp; ++g) {
+            auto f = S.prob.source(t, E.gp_global[g][0], E.gp_global[g][1]);
+            for (int k = 0; k < 3; ++k) src(k, g) = src(k, g) + f[k];
+        }
+    }
+    if (S.prm.friction != RKDG_FRICTION_NONE) {
+        double Cf = S.prm.friction_cf;
+        for (unsigned g = 0; g < ngp; ++g) {
+            double h = E.aux_at_gp(1, g);
+            double u = E.q_at_gp(1, g) / h;
+            double v = E.q_at_gp(2, g) / h;
+            if (E.manning) {
+                Cf = E.g_manning_n_sq / std::pow(h, 1.0 / 3.0);
+                if (Cf < S.prm.friction_cf) Cf = S.prm.friction_cf;
+            }
+            double tau = Cf * std::hypot(u, v) / h;
+            src(1, g) -= tau * E.q_at_gp(1, g);
+            src(2, g) -= tau * E.q_at_gp(2, g);
+        }
+    }
+    Mat T = mul(src, E.int_phi_fact);
+    Mat& rhs = E.rhs[st];
+    for (int k = 0; k < 3; ++k)
+        for (unsigned d = 0; d < ndof; ++d) rhs(k, d) = rhs(k, d) + T(k, d);
+}
+
+static void side_trace(Sim& S, EdgeSide& s) {
+    s.q_at_gp = mul(S.elts[s.elt].q[S.stepper.stage], s.phi);
+}
+
+static void subtract_lift(Sim& S, EdgeSide& s) {
+    Mat T = mul(s.F_hat, s.int_phi_fact);
+    Mat& rhs = S.elts[s.elt].rhs[S.stepper.stage];
+    for (int k = 0; k < 3; ++k)
+        for (int d = 0; d < T.c; ++d) rhs(k, d) = rhs(k, d) - T(k, d);
+}
+
+static void interface_kernel(Sim& S, Interface& I) {  // rkdg_swe_proc_intface.hpp:6-25
+    bool wet_in = S.elts[I.in.elt].wet, wet_ex = S.elts[I.ex.elt].wet;
+    if (!(wet_in || wet_ex)) return;
+    side_trace(S, I.in);
+    side_trace(S, I.ex);
+    unsigned ngp = (unsigned)I.in.int_fact.size();
+    double g = S.prm.g;
+    for (unsigned gp = 0; gp < ngp; ++gp) {  // ISP::Internal::ComputeFlux (rkdg_swe_is_internal.hpp:24-94)
+        unsigned gx = ngp - gp - 1;
+        double qi[3], qe[3], ai[3], ae[3], F[3];
+        for (int k = 0; k < 3; ++k) qi[k] = I.in.q_at_gp(k, gp), qe[k] = I.ex.q_at_gp(k, gx);
+        for (int k = 0; k < 3; ++k) ai[k] = I.in.aux_at_gp(k, gp), ae[k] = I.ex.aux_at_gp(k, gx);
+        LLF_flux(g, qi, qe, ai, I.in.normal[gp].data(), F);
+        for (int k = 0; k < 3; ++k) I.in.F_hat(k, gp) = F[k], I.ex.F_hat(k, gx) = -F[k];
+        if (I.in.F_hat(0, gp) > 1e-12) {
+            if (!wet_in) {
+                for (int k = 0; k < 3; ++k) I.in.F_hat(k, gp) = 0.0;
+                land_flux(g, I.ex.normal[gx].data(), qe, ae, F);
+                for (int k = 0; k < 3; ++k) I.ex.F_hat(k, gx) = F[k];
+            } else if (!wet_ex) {
+                LLF_flux(0.0, qe, qi, ae, I.ex.normal[gx].data(), F);
+                for (int k = 0; k < 3; ++k) I.ex.F_hat(k, gx) = F[k];
+                I.ex.F_hat(0, gx) = -I.in.F_hat(0, gp);
+            }
+        } else if (I.in.F_hat(0, gp) < -1e-12) {
+            if (!wet_ex) {
+                for (int k = 0; k < 3; ++k) I.ex.F_hat(k, gx) = 0.0;
+                land_flux(g, I.in.normal[gp].data(), qi, ai, F);
+                for (int k = 0; k < 3; ++k) I.in.F_hat(k, gp) = F[k];
+            } else if (!wet_in) {
+                LLF_flux(0.0, qi, qe, ai, I.in.normal[gp].data(), F);
+                for (int k = 0; k < 3; ++k) I.in.F_hat(k, gp) = F[k];
+                I.in.F_hat(0, gp) = -I.ex.F_hat(0, gx);
+            }
+        }
+    }
+    subtract_lift(S, I.in);
+    subtract_lift(S, I.ex);
+}
+
+static void boundary_kernel(Sim& S, Boundary& B) {  // rkdg_swe_proc_bound.hpp:6-19
+    Element& E = S.elts[B.s.elt];
+    if (!E.wet) return;
+    side_trace(S, B.s);
+    unsigned ngp = (unsigned)B.s.int_fact.size();
+    double g = S.prm.g, t = S.stepper.time_at_stage();
+    for (unsigned gp = 0; gp < ngp; ++gp) {
+        double qi[3], qe[3], a[3], F[3];
+        for (int k = 0; k < 3; ++k) qi[k] = B.s.q_at_gp(k, gp), a[k] = B.s.aux_at_gp(k, gp);
+        const double* n = B.s.normal[gp].data();
+        if (B.type == RKDG_BC_LAND) {  // rkdg_swe_bc_land.hpp:40-66
+            double n_x = n[0], n_y = n[1], t_x = -n_y, t_y = n_x;
+            double qn_ex = -(qi[1] * n_x + qi[2] * n_y);
+            double qt_ex = qi[1] * t_x + qi[2] * t_y;
+            qe[0] = qi[0], qe[1] = qn_ex * n_x + qt_ex * t_x, qe[2] = qn_ex * n_y + qt_ex * t_y;
+        } else if (B.type == RKDG_BC_TIDE) {  // rkdg_swe_bc_tide.hpp:70-96
+            double ze = 0.0;
+            for (size_t c = 0; c < S.frequency.size(); ++c)
+                ze += S.stepper.ramp * S.forcing_fact[c] * B.amplitude_gp[c][gp] *
+                      std::cos(S.frequency[c] * t + (S.equilib_arg[c] - B.phase_gp[c][gp]) * PI / 180);
+            qe[0] = ze, qe[1] = qi[1], qe[2] = qi[2];
+        } else if (B.type == RKDG_BC_FLOW) {  // rkdg_swe_bc_flow.hpp:72-101
+            double qn = 0.0;
+            for (size_t c = 0; c < S.frequency.size(); ++c)
+                qn += S.stepper.ramp * S.forcing_fact[c] * B.amplitude_gp[c][gp] *
+                      std::cos(S.frequency[c] * t + (S.equilib_arg[c] - B.phase_gp[c][gp]) * PI / 180);
+            qe[0] = qi[0], qe[1] = qn * n[0], qe[2] = qn * n[1];
+        } else {  // Function (rkdg_swe_bc_function.hpp:139-172)
+            auto v = S.prob.ic(t, B.s.gp_global[gp][0], B.s.gp_global[gp][1]);
+            qe[0] = v[0], qe[1] = v[1], qe[2] = v[2];
+        }
+        LLF_flux(g, qi, qe, a, n, F);
+        for (int k = 0; k < 3; ++k) B.s.F_hat(k, gp) = F[k];
+    }
+    subtract_lift(S, B.s);
+}
+
+static void update_kernel(Sim& S, Element& E) {  // stage_serial :41-47, UpdateState (.hpp:67-80)
+    unsigned st = S.stepper.stage, ndof = S.master.ndof;
+    Mat& sol = E.solution[st];
+    const Mat& rhs = E.rhs[st];
+    for (int k = 0; k < 3; ++k)
+        for (unsigned d = 0; d < ndof; ++d) {
+            double s = rhs(k, 0) * (d == 0 ? E.minv[0] : 0.0);
+            for (unsigned j = 1; j < ndof; ++j) s = s + rhs(k, j) * (j == d ? E.minv[j] : 0.0);
+            sol(k, d) = s;
+        }
+    Mat& nq = E.q[st + 1];
+    std::fill(nq.a.begin(), nq.a.end(), 0.0);
+    for (unsigned s = 0; s <= st; ++s) {
+        double a = S.stepper.ark[st][s], b = S.stepper.dt * S.stepper.brk[st][s];
+        for (int k = 0; k < 3; ++k)
+            for (unsigned d = 0; d < ndof; ++d) nq(k, d) += a * E.q[s](k, d) + b * E.solution[s](k, d);
+    }
+    if (st + 1 == S.stepper.nstages) std::swap(E.q[0], E.q[S.stepper.nstages]);
+}
+
+static void wetting_drying_kernel(Sim& S, Element& E) {  // swe_post_wet_dry.hpp:5-145
+    const Master& m = S.master;
+    Mat& q = E.q[S.stepper.stage];
+    const double h_o = S.h_o, thr = S.h_o_threshold;
+    unsigned n_dry = 0;
+    E.went_completely_dry = false;
+    Mat qv = mul(q, m.T_bl);
+    double h[3];
+    for (int v = 0; v < 3; ++v) {
+        h[v] = qv(0, v) + E.bath_at_vrtx[v];
+        if (h[v] <= h_o + thr) ++n_dry;
+    }
+    double h_avg = (((0.0 + h[0]) + h[1]) + h[2]) / 3;
+    if (h_avg <= thr) {
+        E.went_completely_dry = true;
+        h_avg = thr;
+    }
+    bool set_wet = false, set_dry = false, check = false;
+    if (n_dry == 0) {
+        if (E.wet) return;
+        check = true;
+    } else if (n_dry == 3) {
+        for (int v = 0; v < 3; ++v) qv(0, v) = h_avg - E.bath_at_vrtx[v], qv(1, v) = 0.0, qv(2, v) = 0.0;
+        E.wet = false;
+        q = mul(qv, m.T_lb);
+        return;
+    } else {
+        if (h_avg <= h_o + thr) {
+            for (int v = 0; v < 3; ++v) qv(0, v) = h_avg - E.bath_at_vrtx[v], qv(1, v) = 0, qv(2, v) = 0;
+        } else {
+            int hmin = 0, hmax = 0;
+            for (int v = 1; v < 3; ++v) {
+                if (h[v] < h[hmin]) hmin = v;
+                if (h[hmax] < h[v]) hmax = v;
+            }
+            int hmid = 3 - hmax - hmin;
+            double ht[3];
+            ht[hmin] = h_o;
+            ht[hmid] = std::max(h_o, h[hmid] - (ht[hmin] - h[hmin]) / 2);
+            ht[hmax] = h[hmax] - (ht[hmin] - h[hmin]) - (ht[hmid] - h[hmid]);
+            for (int v = 0; v < 3; ++v) h[v] = ht[v];
+            double dqx = 0, dqy = 0;
+            unsigned nd = 0;
+            for (int v = 0; v < 3; ++v)
+                if (h[v] <= h_o + thr) {
+                    nd++;
+                    dqx += qv(1, v), dqy += qv(2, v);
+                    qv(1, v) = 0.0, qv(2, v) = 0.0;
+                }
+            for (int v = 0; v < 3; ++v) {
+                qv(0, v) = h[v] - E.bath_at_vrtx[v];
+                if (h[v] > h_o + thr) {
+                    qv(1, v) += dqx / (3 - nd);
+                    qv(2, v) += dqy / (3 - nd);
+                }
+            }
+        }
+        q = mul(qv, m.T_lb);
+        check = true;
+    }
+    if (check) {
+        int hmax = 0;
+        for (int v = 1; v < 3; ++v)
+            if (h[hmax] < h[v]) hmax = v;
+        double ze_hmax = qv(0, hmax);
+        if (ze_hmax > h_o - E.bath_min)
+            set_wet = true;
+        else
+            set_dry = true;
+    }
+    if (set_dry) {
+        E.wet = false;
+        for (int v = 0; v < 3; ++v) qv(1, v) = 0.0, qv(2, v) = 0.0;
+        q = mul(qv, m.T_lb);
+    } else if (set_wet) {
+        E.wet = true;
+    }
+}
+
+// ---- slope limiter (swe_CS_slope_limiter.hpp, swe_trouble_check.hpp, swe_CS_sl_serial.hpp)
+
+static void R_mat(double g, double h, double u, double v, double nx, double ny, double R[3][3]) {  // swe_jacobian.hpp:462-480
+    double c = std::sqrt(g * h);
+    R[0][0] = 1.0, R[0][1] = 0.0, R[0][2] = 1.0;
+    R[1][0] = u - c * nx, R[1][1] = -ny, R[1][2] = u + c * nx;
+    R[2][0] = v - c * ny, R[2][1] = nx, R[2][2] = v + c * ny;
+}
+
+static void invR_mat(double g, double h, double u, double v, double nx, double ny, double I[3][3]) {  // :531-550
+    double c = std::sqrt(g * h);
+    double un = u * nx + v * ny;
+    I[0][0] = (c + un) / (2 * c), I[0][1] = -nx / (2 * c), I[0][2] = -ny / (2 * c);
+    I[1][0] = u * ny - v * nx, I[1][1] = -ny, I[1][2] = nx;
+    I[2][0] = (c - un) / (2 * c), I[2][1] = nx / (2 * c), I[2][2] = ny / (2 * c);
+}
+
+static void mv3(const double A[3][3], const double* x, double* y) {
+    for (int i = 0; i < 3; ++i) y[i] = A[i][0] * x[0] + A[i][1] * x[1] + A[i][2] * x[2];
+}
+
+static void slope_limiter(Sim& S) {
+    const Master& m = S.master;
+    unsigned st = S.stepper.stage, p = m.p;
+    // prepare element (:8-20)
+    for (auto& E : S.elts) {
+        if (!E.wet) continue;
+        E.sl_q_lin = mul(E.q[st], m.T_bl);
+        for (int k = 0; k < 3; ++k)
+            E.q_at_baryctr[k] = E.sl_q_lin(k, 0) * (1.0 / 3.0) + E.sl_q_lin(k, 1) * (1.0 / 3.0) +
+                                E.sl_q_lin(k, 2) * (1.0 / 3.0);
+        E.sl_q_at_vrtx = E.sl_q_lin;
+        Mat chi(3, 3);  // chi_midpts (master_triangle.tpp:23-33)
+        chi(0, 0) = 0.0, chi(0, 1) = 0.5, chi(0, 2) = 0.5;
+        chi(1, 0) = 0.5, chi(1, 1) = 0.0, chi(1, 2) = 0.5;
+        chi(2, 0) = 0.5, chi(2, 1) = 0.5, chi(2, 2) = 0.0;
+        E.sl_q_at_midpts = mul(E.sl_q_lin, chi);
+    }
+    // prepare interface (:22-35)
+    for (auto& I : S.intfaces) {
+        Element& A = S.elts[I.in.elt];
+        Element& B = S.elts[I.ex.elt];
+        A.wet_neigh[I.in.bound] = B.wet;
+        B.wet_neigh[I.ex.bound] = A.wet;
+        if (A.wet && B.wet) {
+            for (int k = 0; k < 3; ++k)
+                A.q_at_baryctr_neigh[I.in.bound][k] = B.q_at_baryctr[k],
+                B.q_at_baryctr_neigh[I.ex.bound][k] = A.q_at_baryctr[k];
+        }
+    }
+    // prepare boundary (:37-46)
+    for (auto& Bd : S.bounds) {
+        Element& E = S.elts[Bd.s.elt];
+        E.wet_neigh[Bd.s.bound] = E.wet;
+        if (E.wet)
+            for (int k = 0; k < 3; ++k) E.q_at_baryctr_neigh[Bd.s.bound][k] = E.q_at_baryctr[k];
+    }
+    // check_trouble (swe_trouble_check.hpp:5-116)
+    for (auto& E : S.elts) E.I = 0.0, E.perimeter = 0.0, E.troubled = false;
+    for (auto& I : S.intfaces) {
+        Element& A = S.elts[I.in.elt];
+        Element& B = S.elts[I.ex.elt];
+        if (!(A.wet || B.wet)) continue;
+        side_trace(S, I.in);
+        side_trace(S, I.ex);
+        unsigned ngp = (unsigned)I.in.int_fact.size();
+        for (unsigned g = 0; g < ngp; ++g) {
+            I.in.aux_at_gp(1, g) = I.in.q_at_gp(0, g) + I.in.aux_at_gp(0, g);
+            I.ex.aux_at_gp(1, g) = I.ex.q_at_gp(0, g) + I.ex.aux_at_gp(0, g);
+        }
+        std::vector<double> qn_in(ngp), qn_ex(ngp);
+        for (unsigned g = 0; g < ngp; ++g) {  // rows x/y of q_at_gp are ze/qx (:29-35)
+            qn_in[g] = I.in.normal[g][0] * I.in.q_at_gp(0, g) + I.in.normal[g][1] * I.in.q_at_gp(1, g);
+            qn_ex[g] = I.ex.normal[g][0] * I.ex.q_at_gp(0, g) + I.ex.normal[g][1] * I.ex.q_at_gp(1, g);
+        }
+        auto integ = [&](const EdgeSide& s, const std::vector<double>& f) {
+            double r = f[0] * s.int_fact[0];
+            for (unsigned g = 1; g < ngp; ++g) r = r + f[g] * s.int_fact[g];
+            return r;
+        };
+        if (integ(I.in, qn_in) < 0.0) {
+            std::vector<double> hd(ngp);
+            for (unsigned g = 0; g < ngp; ++g) hd[g] = I.in.aux_at_gp(1, g);
+            for (unsigned g = 0; g < ngp; ++g) hd[g] -= I.ex.aux_at_gp(1, ngp - g - 1);
+            A.perimeter += A.lengths[I.in.bound];
+            A.I += std::abs(integ(I.in, hd));
+        }
+        if (integ(I.ex, qn_ex) < 0.0) {
+            std::vector<double> hd(ngp);
+            for (unsigned g = 0; g < ngp; ++g) hd[g] = I.ex.aux_at_gp(1, g);
+            for (unsigned g = 0; g < ngp; ++g) hd[g] -= I.in.aux_at_gp(1, ngp - g - 1);
+            B.perimeter += B.lengths[I.ex.bound];
+            B.I += std::abs(integ(I.ex, hd));
+        }
+    }
+    std::vector<char> in_set(S.elts.size(), 0);
+    for (auto& E : S.elts) {
+        if (!E.wet || E.perimeter == 0.0) continue;
+        double h_norm = 0.0;
+        for (int v = 0; v < 3; ++v) h_norm = std::max(h_norm, E.sl_q_at_vrtx(0, v) + E.sl_bath_at_vrtx[v]);
+        E.I /= (std::pow(E.radius, (p + 1.0) / 2.0) * E.perimeter * h_norm);
+        if (E.I > 1.0) in_set[E.id] = 1;
+    }
+    for (int pass = 0; pass < 6; ++pass) {
+        std::vector<char> nxt(S.elts.size(), 0);
+        for (auto& E : S.elts)
+            if (in_set[E.id]) {
+                E.troubled = true;
+                for (int k = 0; k < 3; ++k)
+                    if (E.neighbor_ID[k] != DEFAULT_ID) nxt[E.neighbor_ID[k]] = 1;
+            }
+        in_set.swap(nxt);
+    }
+    // slope_limiting_kernel (:88-189)
+    const double g = S.prm.g;
+    for (auto& E : S.elts) {
+        E.limited[0] = E.limited[1] = E.limited[2] = false;
+        if (!(E.wet && E.wet_neigh[0] && E.wet_neigh[1] && E.wet_neigh[2])) continue;
+        Mat& q = E.q[st];
+        double h = E.q_at_baryctr[0] + E.bath_at_baryctr;
+        double u = E.q_at_baryctr[1] / h;
+        double v = E.q_at_baryctr[2] / h;
+        double delta[3][3];  // var x bound
+        for (int b = 0; b < 3; ++b) {
+            int e1 = b, e2 = (b + 1) % 3;
+            double mx = E.median[b][0], my = E.median[b][1];
+            double R[3][3], iR[3][3];
+            R_mat(g, h, u, v, mx, my, R);
+            invR_mat(g, h, u, v, mx, my, iR);
+            double qm[3] = {E.sl_q_at_midpts(0, b), E.sl_q_at_midpts(1, b), E.sl_q_at_midpts(2, b)};
+            double wm[3], w0[3], w1[3], w2[3];
+            mv3(iR, qm, wm);
+            mv3(iR, E.q_at_baryctr, w0);
+            mv3(iR, E.q_at_baryctr_neigh[e1], w1);
+            mv3(iR, E.q_at_baryctr_neigh[e2], w2);
+            double wt[3], wd[3], dc[3];
+            for (int k = 0; k < 3; ++k) {
+                wt[k] = wm[k] - w0[k];
+                wd[k] = E.alpha[b][0] * (w1[k] - w0[k]) + E.alpha[b][1] * (w2[k] - w0[k]);
+            }
+            for (int k = 0; k < 3; ++k) {
+                if (std::abs(wt[k]) <= S.M * E.r_sq[b])
+                    dc[k] = wt[k];
+                else if (std::signbit(wt[k]) == std::signbit(wd[k]))
+                    dc[k] = std::copysign(1.0, wt[k]) * std::min(std::abs(wt[k]), std::abs(S.nu * wd[k]));
+                else
+                    dc[k] = 0.0;
+            }
+            double col[3];
+            mv3(R, dc, col);
+            for (int k = 0; k < 3; ++k) delta[k][b] = col[k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            double sum = 0.0;
+            for (int b = 0; b < 3; ++b) sum += delta[k][b];
+            if (sum != 0.0) {
+                double pos = 0.0, neg = 0.0;
+                for (int b = 0; b < 3; ++b) {
+                    pos += std::max(0.0, delta[k][b]);
+                    neg += std::max(0.0, -delta[k][b]);
+                }
+                double tp = std::min(1.0, neg / pos);
+                double tn = std::min(1.0, pos / neg);
+                for (int b = 0; b < 3; ++b)
+                    delta[k][b] = tp * std::max(0.0, delta[k][b]) - tn * std::max(0.0, -delta[k][b]);
+            }
+        }
+        const double T[3][3] = {{-1.0, 1.0, 1.0}, {1.0, -1.0, 1.0}, {1.0, 1.0, -1.0}};
+        for (int k = 0; k < 3; ++k)
+            for (int vv = 0; vv < 3; ++vv) {
+                double s = delta[k][0] * T[0][vv] + delta[k][1] * T[1][vv] + delta[k][2] * T[2][vv];
+                E.sl_q_at_vrtx(k, vv) = E.q_at_baryctr[k] + s;
+            }
+        for (int k = 0; k < 3; ++k) {
+            double d0 = E.sl_q_at_vrtx(k, 0) - E.sl_q_lin(k, 0), d1 = E.sl_q_at_vrtx(k, 1) - E.sl_q_lin(k, 1),
+                   d2 = E.sl_q_at_vrtx(k, 2) - E.sl_q_lin(k, 2);
+            double del_norm = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+            double l0 = E.sl_q_lin(k, 0), l1 = E.sl_q_lin(k, 1), l2 = E.sl_q_lin(k, 2);
+            double q_norm = std::sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+            if (del_norm / q_norm > 1.0e-12) {
+                E.limited[k] = true;
+                for (unsigned d = 0; d < m.ndof; ++d)
+                    q(k, d) = E.sl_q_at_vrtx(k, 0) * m.T_lb(0, d) + E.sl_q_at_vrtx(k, 1) * m.T_lb(1, d) +
+                              E.sl_q_at_vrtx(k, 2) * m.T_lb(2, d);
+            }
+        }
+    }
+}
+
+static bool scrutinize(Sim& S, Element& E) {  // swe_post_scrutinize.hpp:5-41
+    const Mat& q = E.q[S.stepper.stage];
+    for (int k = 0; k < 3; ++k)
+        for (unsigned d = 0; d < S.master.ndof; ++d)
+            if (std::isnan(q(k, d))) return true;
+    return false;
+}
+
+// stage_serial (rkdg_swe_proc_serial_step.hpp:28-67); with nthreads > 1 the element loops run
+// under OpenMP and the interface loop colour by colour (the OMPI analogue, for timing only).
+static int stage(Sim& S) {
+    const int ne = (int)S.elts.size();
+    if (S.nthreads <= 1) {
+        for (auto& E : S.elts) volume_kernel(S, E);
+        for (auto& E : S.elts) source_kernel(S, E);
+        for (auto& I : S.intfaces) interface_kernel(S, I);
+        for (auto& B : S.bounds) boundary_kernel(S, B);
+        for (auto& E : S.elts) update_kernel(S, E);
+    } else {
+#pragma omp parallel for num_threads(S.nthreads) schedule(static)
+        for (int e = 0; e < ne; ++e) {
+            volume_kernel(S, S.elts[e]);
+            source_kernel(S, S.elts[e]);
+        }
+        for (auto& colour : S.intface_colours) {
+            const int nc = (int)colour.size();
+#pragma omp parallel for num_threads(S.nthreads) schedule(static)
+            for (int i = 0; i < nc; ++i) interface_kernel(S, S.intfaces[colour[i]]);
+        }
+        for (auto& B : S.bounds) boundary_kernel(S, B);
+#pragma omp parallel for num_threads(S.nthreads) schedule(static)
+        for (int e = 0; e < ne; ++e) update_kernel(S, S.elts[e]);
+    }
+    unsigned old_stage = S.stepper.stage;
+    unsigned old_step = S.stepper.step;
+    S.stepper.advance();
+    if (S.prm.wetting_drying) {
+#pragma omp parallel for num_threads(S.nthreads) schedule(static) if (S.nthreads > 1)
+        for (int e = 0; e < ne; ++e) wetting_drying_kernel(S, S.elts[e]);
+    }
+    if (S.prm.slope_limiting) slope_limiter(S);
+    for (auto& E : S.elts)
+        if (scrutinize(S, E)) {
+            S.nan_elt = (int)E.id, S.nan_step = (int)old_step, S.nan_stage = (int)old_stage;
+            return RKDG_ERR_NAN;
+        }
+    return RKDG_OK;
+}
+
+static double l2_residual(Sim& S) {  // simulation.hpp:85-93, element.hpp:547-592
+    Rule2D rule = dunavant(20);
+    Mat phi;
+    dubiner(S.master.p, rule.z, &phi, nullptr, nullptr);
+    double t = S.stepper.time_at_stage();
+    double res = 0;
+    for (auto& E : S.elts) {
+        Mat ugp = mul(E.q[0], phi);  // compute_residual_L2 reads state[0] (swe_post_comp_res_l2.hpp:7-14)
+        auto gg = E.shape.local_to_global(rule.z);
+        size_t ngp = rule.w.size();
+        double detabs = std::abs(E.shape.detJ());
+        double L2 = 0;
+        double l2v[3];
+        for (int k = 0; k < 3; ++k) {
+            double s = 0;
+            for (size_t g = 0; g < ngp; ++g) {
+                auto tv = S.prob.true_q(t, gg[g][0], gg[g][1]);
+                double d = tv[k] - ugp(k, (int)g);
+                double term = (d * d) * rule.w[g];
+                s = (g == 0) ? term : s + term;
+            }
+            l2v[k] = s * detabs;
+        }
+        for (int k = 0; k < 3; ++k) L2 += l2v[k];
+        res += L2;
+    }
+    return std::sqrt(res);
+}
+
+}  // namespace orc
+
+// ==========================================================================================
+// C API (loaded with ctypes by the tests and the benchmark's CPU-baseline leg).
+extern "C" {
+
+void* orc_create(const rkdg_mesh_desc* md, const rkdg_params* prm) { return orc::create(md, prm); }
+const char* orc_error() { return orc::g_err; }
+void orc_destroy(void* h) { delete (orc::Sim*)h; }
+void orc_set_threads(void* h, int n) { ((orc::Sim*)h)->nthreads = n; }
+int orc_max_threads() {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int orc_stage(void* h) { return orc::stage(*(orc::Sim*)h); }
+int orc_step(void* h, int nsteps) {
+    orc::Sim& S = *(orc::Sim*)h;
+    for (int s = 0; s < nsteps; ++s)
+        for (unsigned k = 0; k < S.stepper.nstages; ++k) {
+            int r = orc::stage(S);
+            if (r) return r;
+        }
+    return 0;
+}
+void orc_nan_info(void* h, int* elt, int* step, int* stage) {
+    orc::Sim& S = *(orc::Sim*)h;
+    *elt = S.nan_elt, *step = S.nan_step, *stage = S.nan_stage;
+}
+
+void orc_get_state(void* h, double* q, uint8_t* wet, uint8_t* limited, uint8_t* troubled) {
+    orc::Sim& S = *(orc::Sim*)h;
+    size_t ne = S.elts.size(), ndof = S.master.ndof;
+    for (size_t e = 0; e < ne; ++e) {
+        const orc::Element& E = S.elts[e];
+        if (q)
+            for (int k = 0; k < 3; ++k)
+                for (size_t d = 0; d < ndof; ++d) q[(k * ndof + d) * ne + e] = E.q[S.stepper.stage](k, (int)d);
+        if (wet) wet[e] = E.wet;
+        if (limited)
+            for (int k = 0; k < 3; ++k) limited[k * ne + e] = E.limited[k];
+        if (troubled) troubled[e] = E.troubled;
+    }
+}
+
+void orc_set_state(void* h, const double* q, const uint8_t* wet) {
+    orc::Sim& S = *(orc::Sim*)h;
+    size_t ne = S.elts.size(), ndof = S.master.ndof;
+    for (size_t e = 0; e < ne; ++e) {
+        orc::Element& E = S.elts[e];
+        if (q)
+            for (int k = 0; k < 3; ++k)
+                for (size_t d = 0; d < ndof; ++d) E.q[S.stepper.stage](k, (int)d) = q[(k * ndof + d) * ne + e];
+        if (wet) E.wet = wet[e] != 0;
+    }
+}
+
+void orc_get_time(void* h, double* t, int* step, int* stage) {
+    orc::Sim& S = *(orc::Sim*)h;
+    *t = S.stepper.t, *step = (int)S.stepper.step, *stage = (int)S.stepper.stage;
+}
+
+double orc_l2_residual(void* h) { return orc::l2_residual(*(orc::Sim*)h); }
+
+int orc_num_interfaces(void* h) { return (int)((orc::Sim*)h)->intfaces.size(); }
+int orc_num_boundaries(void* h) { return (int)((orc::Sim*)h)->bounds.size(); }
+
+// ---- unit-level entry points for the golden-vector tests --------------------------------
+
+int orc_dubiner(int p, int npt, const double* pts, double* phi, double* dphi1, double* dphi2) {
+    std::vector<std::array<double, 2>> P(npt);
+    for (int k = 0; k < npt; ++k) P[k] = {pts[2 * k], pts[2 * k + 1]};
+    orc::Mat a, b, c;
+    orc::dubiner((unsigned)p, P, &a, &b, &c);
+    std::memcpy(phi, a.a.data(), a.a.size() * sizeof(double));
+    std::memcpy(dphi1, b.a.data(), b.a.size() * sizeof(double));
+    std::memcpy(dphi2, c.a.data(), c.a.size() * sizeof(double));
+    return a.r;
+}
+
+int orc_dunavant(int p, double* w, double* pts) {
+    orc::Rule2D R = orc::dunavant((unsigned)p);
+    for (size_t g = 0; g < R.w.size(); ++g) {
+        if (w) w[g] = R.w[g];
+        if (pts) pts[2 * g] = R.z[g][0], pts[2 * g + 1] = R.z[g][1];
+    }
+    return (int)R.w.size();
+}
+
+int orc_gauss_legendre(int p, double* w, double* z) {
+    orc::Rule1D R = orc::gauss_legendre((unsigned)p);
+    for (size_t g = 0; g < R.w.size(); ++g) {
+        if (w) w[g] = R.w[g];
+        if (z) z[g] = R.z[g];
+    }
+    return (int)R.w.size();
+}
+
+void orc_llf_flux(double g, const double* q_in, const double* q_ex, const double* aux, const double* n, double* F) {
+    orc::LLF_flux(g, q_in, q_ex, aux, n, F);
+}
+
+// ark/brk: [nstages][nstages] row-major (zero beyond the ragged row); drk: [nstages]
+int orc_rk_coeffs(int nstages, int order, double* ark, double* brk, double* drk) {
+    orc::Stepper st;
+    if (!st.init(nstages, order, 1.0, 0.0)) return 1;
+    for (int i = 0; i < nstages; ++i)
+        for (int k = 0; k < nstages; ++k) {
+            ark[i * nstages + k] = k <= i ? st.ark[i][k] : 0.0;
+            brk[i * nstages + k] = k <= i ? st.brk[i][k] : 0.0;
+        }
+    for (int i = 0; i < nstages; ++i) drk[i] = st.drk[i];
+    return 0;
+}
+
+// Stage times seen by the stepper over nsteps (t_stage per stage, flattened).
+int orc_stage_times(int nstages, int order, double dt, int nsteps, double* out) {
+    orc::Stepper st;
+    if (!st.init(nstages, order, dt, 0.0)) return 1;
+    for (int s = 0; s < nsteps * nstages; ++s) {
+        out[s] = st.time_at_stage();
+        st.advance();
+    }
+    return 0;
+}
+
+// Element integration operators for the reference's element golden test
+// (test/test_element_triangle.{cpp,hpp}): given f at the element's rule-2p points,
+// return Integration, IntegrationPhi[ndof], IntegrationDPhi x/y [ndof].
+int orc_element_integrals(int p, const double* vx, const double* vy, const double* f_gp, double* integ,
+                          double* iphi, double* idx, double* idy) {
+    rkdg_mesh_desc md{};
+    double xyz[9];
+    for (int k = 0; k < 3; ++k) xyz[3 * k] = vx[k], xyz[3 * k + 1] = vy[k], xyz[3 * k + 2] = 1.0;
+    int en[3] = {0, 1, 2};
+    int be[6] = {1, 2, 2, 0, 0, 1};
+    uint8_t bt[3] = {0, 0, 0};
+    md.n_nodes = 3, md.n_elements = 1, md.node_xyz = xyz, md.elt_nodes = en;
+    md.n_bound_edges = 3, md.bound_edge_nodes = be, md.bound_edge_type = bt;
+    rkdg_params prm{};
+    prm.p = p, prm.nstages = 1, prm.order = 1, prm.dt = 1, prm.g = 9.81, prm.ic_constant = 1;
+    orc::Sim* S = orc::create(&md, &prm);
+    if (!S) return 1;
+    orc::Element& E = S->elts[0];
+    unsigned ngp = S->master.ngp, ndof = S->master.ndof;
+    double s = f_gp[0] * E.int_fact[0];
+    for (unsigned g = 1; g < ngp; ++g) s = s + f_gp[g] * E.int_fact[g];
+    *integ = s;
+    for (unsigned d = 0; d < ndof; ++d) {
+        double a = f_gp[0] * E.int_phi_fact(0, d), b = f_gp[0] * E.int_dphi_fact[0](0, d),
+               c = f_gp[0] * E.int_dphi_fact[1](0, d);
+        for (unsigned g = 1; g < ngp; ++g) {
+            a = a + f_gp[g] * E.int_phi_fact(g, d);
+            b = b + f_gp[g] * E.int_dphi_fact[0](g, d);
+            c = c + f_gp[g] * E.int_dphi_fact[1](g, d);
+        }
+        iphi[d] = a, idx[d] = b, idy[d] = c;
+    }
+    delete S;
+    return 0;
+}
+
+// Global coordinates of the element's rule-2p points (for building f_gp in the test).
+int orc_element_gp_global(int p, const double* vx, const double* vy, double* xy) {
+    orc::Shape sh;
+    for (int k = 0; k < 3; ++k) sh.x[k] = vx[k], sh.y[k] = vy[k], sh.z[k] = 0;
+    orc::Rule2D R = orc::dunavant(2 * (unsigned)p);
+    auto g = sh.local_to_global(R.z);
+    for (size_t k = 0; k < g.size(); ++k) xy[2 * k] = g[k][0], xy[2 * k + 1] = g[k][1];
+    return (int)g.size();
+}
+
+// One wetting-drying kernel call on a single p=1 element (test/test_wetting_drying.cpp):
+// the state is the nodal projection of q_vrtx [3 vars][3 vertices]; on return q_vrtx holds
+// the vertex values of the new state and flags/h_at_vrtx are reported.
+int orc_wet_dry_single(const double* vx, const double* vy, const double* bath_vrtx, double bath_min, double h_o,
+                       int wet_in, double* q_vrtx, int* wet_out, int* went_dry, double* h_at_vrtx) {
+    rkdg_mesh_desc md{};
+    double xyz[9];
+    for (int k = 0; k < 3; ++k) xyz[3 * k] = vx[k], xyz[3 * k + 1] = vy[k], xyz[3 * k + 2] = 0.0;
+    int en[3] = {0, 1, 2};
+    int be[6] = {1, 2, 2, 0, 0, 1};
+    uint8_t bt[3] = {0, 0, 0};
+    md.n_nodes = 3, md.n_elements = 1, md.node_xyz = xyz, md.elt_nodes = en;
+    md.n_bound_edges = 3, md.bound_edge_nodes = be, md.bound_edge_type = bt;
+    rkdg_params prm{};
+    prm.p = 1, prm.nstages = 1, prm.order = 1, prm.dt = 1, prm.g = 9.81, prm.ic_constant = 1;
+    orc::Sim* S = orc::create(&md, &prm);
+    if (!S) return 1;
+    S->h_o = h_o;  // the reference test sets h_o but leaves h_o_threshold at its default 1e-6
+    S->h_o_threshold = 1.0e-6;
+    orc::Element& E = S->elts[0];
+    for (int v = 0; v < 3; ++v) E.bath_at_vrtx[v] = bath_vrtx[v];
+    E.bath_min = bath_min;
+    E.wet = wet_in != 0;
+    orc::Mat qv(3, 3);
+    for (int k = 0; k < 3; ++k)
+        for (int v = 0; v < 3; ++v) qv(k, v) = q_vrtx[3 * k + v];
+    orc::Mat psi = orc::Shape::psi(S->master.rule.z);
+    orc::Mat Minv(S->master.ndof, S->master.ndof);
+    for (unsigned d = 0; d < S->master.ndof; ++d) Minv(d, d) = E.minv[d];
+    E.q[0] = orc::mul(orc::mul(orc::mul(qv, psi), E.int_phi_fact), Minv);
+    orc::wetting_drying_kernel(*S, E);
+    orc::Mat out = orc::mul(E.q[0], S->master.T_bl);
+    for (int k = 0; k < 3; ++k)
+        for (int v = 0; v < 3; ++v) q_vrtx[3 * k + v] = out(k, v);
+    *wet_out = E.wet, *went_dry = E.went_completely_dry;
+    // h at vertices as the kernel left them (recomputed from the returned linear state)
+    for (int v = 0; v < 3; ++v) h_at_vrtx[v] = out(0, v) + E.bath_at_vrtx[v];
+    delete S;
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,200 @@
+"""Oracle LLF flux, element integration, wetting-drying and SSP-RK stepper against the
+reference's unit tests (golden vectors and scenarios) and closed forms.
+
+  LLF: test/test_llf_flux.cpp (3 configurations, 100 eps)
+  element integrals: test/test_element_triangle.{cpp,hpp} (p = 10, 1e4 eps)
+  wetting-drying: the five scenarios of test/test_wetting_drying.cpp
+  stepper: y'' + y = t/2 accuracy (test/test_rk_stepper.cpp) and the drk = 0 stage-time quirk
+"""
+import ctypes as C
+import json
+import math
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle.oracle_py import load
+
+GOLD = Path(__file__).resolve().parent / "golden"
+EPS = np.finfo(float).eps
+dp = C.POINTER(C.c_double)
+
+
+def almost_equal(a, b, factor=100.0):  # utilities/almost_equal.hpp:16-22
+    m = max(abs(a), abs(b))
+    if m < EPS * factor:
+        return True
+    return abs(a - b) < m * EPS * factor
+
+
+def arr(x):
+    return np.ascontiguousarray(x, dtype=np.float64)
+
+
+def llf(g, qi, qe, bath, sp, n):
+    lib = load()
+    F = np.empty(3)
+    aux = arr([bath, qi[0] + bath, sp])
+    lib.orc_llf_flux(g, arr(qi).ctypes.data_as(dp), arr(qe).ctypes.data_as(dp), aux.ctypes.data_as(dp),
+                     arr(n).ctypes.data_as(dp), F.ctypes.data_as(dp))
+    return F
+
+
+def test_llf_golden():
+    d = json.load(open(GOLD / "llf_flux.json"))
+    for c in d["cases"]:
+        n = [c["normal_sign"][0] / math.sqrt(2.0), c["normal_sign"][1] / math.sqrt(2.0)]
+        F = llf(d["g"], c["q_in"], c["q_ex"], c["bath"], c["sp"], n)
+        for k in range(3):
+            assert almost_equal(F[k], c["F_hat"][k], d["tol_factor"]), (c, F)
+
+
+def test_llf_closed_form_consistency():
+    # identical states: F_hat = F(q).n exactly (the dissipation term vanishes)
+    q = [0.3, 1.2, -0.7]
+    b, g = 5.0, 9.81
+    n = [0.6, 0.8]
+    F = llf(g, q, q, b, 1.0, n)
+    h = q[0] + b
+    u, v = q[1] / h, q[2] / h
+    pe = g * (q[0] ** 2 / 2 + q[0] * b)
+    ref = [q[1] * n[0] + q[2] * n[1], (u * q[1] + pe) * n[0] + u * q[2] * n[1], u * q[2] * n[0] + (v * q[2] + pe) * n[1]]
+    assert np.allclose(F, ref, rtol=1e-15, atol=1e-15)
+    # conservation: swapping sides and reversing the normal negates the flux
+    F1 = llf(g, [0.1, 0.5, 0.2], [0.3, -0.1, 0.4], b, 1.0, n)
+    F2 = llf(g, [0.3, -0.1, 0.4], [0.1, 0.5, 0.2], b, 1.0, [-n[0], -n[1]])
+    assert np.allclose(F1, -F2, rtol=1e-14)
+
+
+def test_element_integrals_golden():
+    d = json.load(open(GOLD / "element_p10.json"))
+    lib = load()
+    vx, vy = arr([-0.5, 0.5, 0.0]), arr([0.0, 0.0, math.sqrt(3.0) / 2.0])
+    xy = np.empty(2 * 100)
+    n = lib.orc_element_gp_global(10, vx.ctypes.data_as(dp), vy.ctypes.data_as(dp), xy.ctypes.data_as(dp))
+    xy = xy[:2 * n].reshape(n, 2)
+    f = arr((xy[:, 0] + 1.0) ** 2 + (xy[:, 1] - 1.0) ** 2)
+    integ = C.c_double()
+    iphi, idx, idy = np.empty(66), np.empty(66), np.empty(66)
+    assert lib.orc_element_integrals(10, vx.ctypes.data_as(dp), vy.ctypes.data_as(dp), f.ctypes.data_as(dp),
+                                     C.byref(integ), iphi.ctypes.data_as(dp), idx.ctypes.data_as(dp),
+                                     idy.ctypes.data_as(dp)) == 0
+    tol = d["tol_factor"]
+    assert almost_equal(d["IntegrationPhi_true"][0], integ.value, tol)
+    for k in range(66):
+        assert almost_equal(d["IntegrationPhi_true"][k], iphi[k], tol), k
+        assert almost_equal(d["IntegrationDPhiDX_true"][k], idx[k], tol), k
+        assert almost_equal(d["IntegrationDPhiDY_true"][k], idy[k], tol), k
+
+
+def test_wetting_drying_scenarios():
+    d = json.load(open(GOLD / "wetting_drying.json"))
+    lib = load()
+    h_o, b = d["h_o"], d["depth"]
+    vx, vy = arr([-0.5, 0.5, 0.0]), arr([0.0, 0.0, math.sqrt(3.0) / 2.0])
+    bv = arr(b)
+
+    def val(e, v):
+        return e[0] * h_o - b[v] if len(e) == 2 else e[0] * h_o
+
+    wet = True
+    for sc in d["scenarios"]:
+        if sc["wet_in"] is not None:
+            wet = sc["wet_in"]
+        qv = arr([[val(sc["ze_in"][v], v) for v in range(3)], d["qx_in"], d["qy_in"]])
+        wet_out, went_dry = C.c_int32(), C.c_int32()
+        hv = np.empty(3)
+        assert lib.orc_wet_dry_single(vx.ctypes.data_as(dp), vy.ctypes.data_as(dp), bv.ctypes.data_as(dp),
+                                      d["bath_min"], h_o, int(wet), qv.ctypes.data_as(dp), C.byref(wet_out),
+                                      C.byref(went_dry), hv.ctypes.data_as(dp)) == 0
+        qv = qv.reshape(3, 3)
+        assert bool(wet_out.value) == sc["wet"], sc["name"]
+        wet = bool(wet_out.value)
+        tz, tq = sc["tol"]
+        if sc["ze"] == "h_avg - b":
+            # h_avg of the kernel's vertex depths before flattening: the inputs' mean
+            hin = [val(sc["ze_in"][v], v) + b[v] for v in range(3)]
+            h_avg = (hin[0] + hin[1] + hin[2]) / 3
+            ze = [h_avg - b[v] for v in range(3)]
+        else:
+            ze = [val(sc["ze"][v], v) for v in range(3)]
+        for v in range(3):
+            assert almost_equal(qv[0, v], ze[v], tz), (sc["name"], v)
+            assert almost_equal(qv[1, v], sc["qx"][v], tq), (sc["name"], v)
+            assert almost_equal(qv[2, v], sc["qy"][v], tq), (sc["name"], v)
+
+
+def rk_coeffs_oracle(ns, order):
+    lib = load()
+    a, b, dd = np.zeros(ns * ns), np.zeros(ns * ns), np.zeros(ns)
+    assert lib.orc_rk_coeffs(ns, order, a.ctypes.data_as(dp), b.ctypes.data_as(dp), dd.ctypes.data_as(dp)) == 0
+    return a.reshape(ns, ns), b.reshape(ns, ns), dd
+
+
+def rk_coeffs_engine(ns, order):
+    from dgswemv2_amd import _native as N
+    lib = N.load_library()
+    a, b, dd = np.zeros(ns * ns), np.zeros(ns * ns), np.zeros(ns)
+    assert lib.rkdg_ssprk_coeffs(ns, order, N.dptr(a), N.dptr(b), N.dptr(dd)) == 0
+    return a.reshape(ns, ns), b.reshape(ns, ns), dd
+
+
+def ode_error(a, b, dd, dt=5e-5, T=5.0):
+    """test/test_rk_stepper.cpp: y'' + y = t/2, y(0) = 0, y'(0) = 1.5 -> sin t + t/2."""
+    ns = a.shape[0]
+    nsteps = int(T / dt) + 1
+    y = np.array([0.0, 1.5])
+    t = 0.0
+    for _ in range(nsteps):
+        ys = [y]
+        rhs = []
+        for s in range(ns):
+            ts = t + dt * dd[s]
+            rhs.append(np.array([ys[s][1], -ys[s][0] + 0.5 * ts]))
+            nxt = np.zeros(2)
+            for k in range(s + 1):
+                nxt = nxt + a[s, k] * ys[k] + dt * b[s, k] * rhs[k]
+            ys.append(nxt)
+        y = ys[ns]
+        t += dt
+    return abs(y[0] - (math.sin(t) + t / 2)), abs(y[1] - (math.cos(t) + 0.5))
+
+
+@pytest.mark.parametrize("ns,order", [(1, 1), (2, 2), (3, 3), (4, 3)])
+def test_oracle_stepper_ode(ns, order):
+    e0, e1 = ode_error(*rk_coeffs_oracle(ns, order), dt=1e-3)
+    assert e0 < 1e-2 and e1 < 1e-2
+
+
+@pytest.mark.parametrize("ns,order", [(1, 1), (2, 2), (3, 3), (4, 3), (5, 3), (6, 3), (7, 3), (8, 3), (5, 4), (6, 4),
+                                      (7, 4), (8, 4)])
+def test_engine_stepper_tables(ns, order):
+    a, b, dd = rk_coeffs_engine(ns, order)
+    # consistency: each stage is a convex-like combination (rows of alpha sum to 1)
+    assert np.allclose(a.sum(axis=1), 1.0, atol=1e-12)
+    e0, e1 = ode_error(a, b, dd, dt=1e-3)
+    assert e0 < 1e-2 and e1 < 1e-2
+    if ns <= 4:
+        ao, bo, do = rk_coeffs_oracle(ns, order)
+        assert np.array_equal(a, ao) and np.array_equal(b, bo) and np.array_equal(dd, do)
+
+
+@pytest.mark.parametrize("ns,order", [(2, 2), (3, 3)])
+def test_stage_time_quirk(ns, order):
+    # drk is built from crk entries whose diagonal is never set: every stage sees t_n
+    lib = load()
+    out = np.empty(5 * ns)
+    lib.orc_stage_times(ns, order, 0.25, 5, out.ctypes.data_as(dp))
+    expect = np.repeat(0.25 * np.arange(5), ns)
+    assert np.array_equal(out, expect)
+    assert np.all(rk_coeffs_engine(ns, order)[2] == 0.0)
+
+
+def test_rk_order_of_accuracy():
+    # second-order scheme: error ratio 4 when dt halves (with the true stage times of an ODE
+    # without explicit t-dependence the drk quirk is invisible)
+    a, b, dd = rk_coeffs_engine(2, 2)
+    e1 = ode_error(a, b, dd, dt=0.02, T=2.0)[0]
+    e2 = ode_error(a, b, dd, dt=0.01, T=2.0)[0]
+    assert 1.5 < e1 / e2 < 5.0
