@@ -8,6 +8,7 @@
 #include <climits>
 #include <cmath>
 #include <limits>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -52,7 +53,6 @@ struct Engine final : EngineBase {
     uint8_t* wet[2] = {nullptr, nullptr};
     int wcur = 0;
     double* bath_vrtx = nullptr;
-    void* tables = nullptr;
     unsigned long long* d_nan = nullptr;
     // limiter
     double *median = nullptr, *alpha = nullptr, *rsq = nullptr, *len = nullptr, *radius = nullptr,
@@ -62,7 +62,10 @@ struct Engine final : EngineBase {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> stage_ev, lim_ev;
 
-    Engine(rkdg::Model& m, hipStream_t s) : M(m), stream(s) {}
+    Engine(rkdg::Model& m, hipStream_t s) : M(m), stream(s) {
+        if (const char* b = std::getenv("RKDG_BLOCK")) block = std::atoi(b);  // tuning experiments
+        if (block < 64 || block > 256 || block % 64) block = 256;
+    }
     ~Engine() override {
         for (auto& pr : stage_ev) hipEventDestroy(pr.first), hipEventDestroy(pr.second);
         for (auto& pr : lim_ev) hipEventDestroy(pr.first), hipEventDestroy(pr.second);
@@ -90,49 +93,6 @@ struct Engine final : EngineBase {
         std::vector<T> tmp(h.begin(), h.end());
         hipError_t e = hipMemcpy(p, tmp.data(), tmp.size() * sizeof(T), hipMemcpyHostToDevice);
         return e == hipSuccess ? true : fail("hipMemcpy", e);
-    }
-
-    template <int P>
-    bool upload_tables() {
-        using Tb = rkdg::Tables<Real, P>;
-        constexpr int ND = Tb::ND, NG = Tb::NG, NE = Tb::NE;
-        const rkdg::Master& m = M.master;
-        if (m.ndof != ND || m.ngp != NG || m.ngp_e != NE) {
-            err = "master table sizes do not match the kernel build";
-            return false;
-        }
-        Tb T;
-        std::memset(&T, 0, sizeof T);
-        for (int d = 0; d < ND; ++d)
-            for (int g = 0; g < NG; ++g) {
-                T.phi[d][g] = (Real)m.phi[d * NG + g];
-                T.dphi[0][d][g] = (Real)m.dphi[0][d * NG + g];
-                T.dphi[1][d][g] = (Real)m.dphi[1][d * NG + g];
-                T.ipf[g][d] = (Real)m.ipf[g * ND + d];
-                T.idphi[0][g][d] = (Real)m.idphi[0][g * ND + d];
-                T.idphi[1][g][d] = (Real)m.idphi[1][g * ND + d];
-            }
-        for (int d = 0; d < ND; ++d) T.minv[d] = (Real)m.minv[d];
-        for (int f = 0; f < 3; ++f)
-            for (int d = 0; d < ND; ++d)
-                for (int g = 0; g < NE; ++g) T.phie[f][d][g] = (Real)m.phi_e[f][d * NE + g];
-        for (int g = 0; g < NE; ++g) T.we[g] = (Real)m.erule.w[g];
-        for (int g = 0; g < NG; ++g) {
-            const double z1 = m.rule.z[g][0], z2 = m.rule.z[g][1];
-            T.psi[0][g] = (Real)(-(z1 + z2) / 2), T.psi[1][g] = (Real)((1 + z1) / 2), T.psi[2][g] = (Real)((1 + z2) / 2);
-        }
-        for (int f = 0; f < 3; ++f)
-            for (int g = 0; g < NE; ++g) {
-                const double z1 = m.zedge[f][g][0], z2 = m.zedge[f][g][1];
-                T.psie[f][0][g] = (Real)(-(z1 + z2) / 2);
-                T.psie[f][1][g] = (Real)((1 + z1) / 2);
-                T.psie[f][2][g] = (Real)((1 + z2) / 2);
-            }
-        char* d = nullptr;
-        if (!alloc(d, sizeof T)) return false;
-        tables = d;
-        hipError_t e = hipMemcpy(d, &T, sizeof T, hipMemcpyHostToDevice);
-        return e == hipSuccess ? true : fail("hipMemcpy tables", e);
     }
 
     rkdg_status init() override {
@@ -166,8 +126,6 @@ struct Engine final : EngineBase {
         if (!alloc(d_nan, 1)) return RKDG_ERR_HIP;
         if (hipMemcpy(d_nan, &kNoNan, sizeof kNoNan, hipMemcpyHostToDevice) != hipSuccess)
             return fail("hipMemcpy", hipGetLastError()), RKDG_ERR_HIP;
-        const bool ok = M.p == 1 ? upload_tables<1>() : upload_tables<2>();
-        if (!ok) return RKDG_ERR_HIP;
         std::vector<Real> q0(M.q0.begin(), M.q0.end());
         if (hipMemcpy(q[0], q0.data(), q0.size() * sizeof(Real), hipMemcpyHostToDevice) != hipSuccess)
             return fail("hipMemcpy state", hipGetLastError()), RKDG_ERR_HIP;
@@ -226,7 +184,7 @@ struct Engine final : EngineBase {
 
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) e0 = take_event(), hipEventRecord(e0, stream);
-        hipError_t e = rkdg::launch_stage<Real>(M.p, M.prm.wetting_drying != 0, A, tables, stream, block);
+        hipError_t e = rkdg::launch_stage<Real>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block);
         if (e != hipSuccess) return fail("stage kernel launch", e), RKDG_ERR_HIP;
         if (timing) e1 = take_event(), hipEventRecord(e1, stream), stage_ev.push_back({e0, e1});
         ++stages_timed;
@@ -250,7 +208,7 @@ struct Engine final : EngineBase {
             L.nan_check = 1, L.nan_key = d_nan, L.stage_key = key;
             hipEvent_t l0 = nullptr, l1 = nullptr;
             if (timing) l0 = take_event(), hipEventRecord(l0, stream);
-            e = rkdg::launch_limiter<Real>(M.p, L, tables, stream, block);
+            e = rkdg::launch_limiter<Real>(M.p, L, stream, block);
             if (e != hipSuccess) return fail("limiter kernel launch", e), RKDG_ERR_HIP;
             if (timing) l1 = take_event(), hipEventRecord(l1, stream), lim_ev.push_back({l0, l1});
         }
@@ -372,7 +330,7 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
     c->precision = precision;
     std::string msg;
     try {
-        msg = rkdg::build_model(*mesh, *params, c->model);
+        msg = rkdg::build_model(*mesh, *params, c->model, precision != RKDG_FP32_BULK);
     } catch (const std::exception& ex) {
         msg = ex.what();
     }

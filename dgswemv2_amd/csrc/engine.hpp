@@ -25,7 +25,10 @@
 #define RKDG_HD
 #endif
 
+
 #include <cmath>
+
+#include "device_tables.h"
 
 namespace rkdg {
 
@@ -36,21 +39,6 @@ struct Sizes {
     static constexpr int ND = (P + 1) * (P + 2) / 2;            // modes
     static constexpr int NG = P == 1 ? 3 : (P == 2 ? 6 : 12);   // Dunavant rule 2P points
     static constexpr int NE = P + 1;                            // Gauss-Legendre rule 2P+1 points
-};
-
-// Reference-element tables in the arithmetic type of the kernel (master_triangle.tpp:5-87).
-template <typename Real, int P>
-struct Tables {
-    static constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
-    Real phi[ND][NG];
-    Real dphi[2][ND][NG];
-    Real idphi[2][NG][ND];
-    Real ipf[NG][ND];
-    Real minv[ND];
-    Real phie[3][ND][NE];
-    Real we[NE];
-    Real psi[3][NG];        // shape functions at the rule points (gp global coordinates)
-    Real psie[3][3][NE];    // [face][node][gp] shape functions at the edge points
 };
 
 // Everything one stage launch reads and writes.  Arrays are element-fastest SoA.
@@ -318,19 +306,93 @@ RKDG_HD inline void from_vertices(const Real* v, Real* q) {
 }
 
 // ---------------------------------------------------------------------------------------
-// The fused stage of one element.
+// Face tables indexed by a per-lane local face: phie[(f*ND + d)*NE + g], psie[(f*3 + node)*NE + g].
+// The kernels stage them in LDS; the remaining reference-element tables are compile-time
+// constants (ConstTables, generated from tables.cpp at build time).
+template <typename Real>
+struct FaceTables {
+    const Real* phie;
+    const Real* psie;
+};
 
+// Tide/flow exterior forcing value at one edge point (rkdg_swe_bc_tide.hpp:76-83).
+template <typename Real, int NE>
+RKDG_HD inline Real harmonic_forcing(const StageArgs<Real>& A, int ed, int g) {
+    const int fr = A.eforce[ed];
+    Real acc = Real(0.0);
+    for (int c = 0; c < A.ncon; ++c) {
+        using std::cos;
+        const Real amp = A.famp[((size_t)fr * A.ncon + c) * NE + g];
+        const Real ph = A.fphase[((size_t)fr * A.ncon + c) * NE + g];
+        acc += Real(A.ramp) * A.ffact[c] * amp * cos(A.freq[c] * Real(A.t) + (A.eqarg[c] - ph) * Real(kPI) / Real(180));
+    }
+    return acc;
+}
+
+// Wetting-drying of the new state (swe_post_wet_dry.hpp:5-145, FP64), store, NaN scan.
 template <typename Real, int P, bool WD>
-RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P>& T, int e) {
+RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Real (&nq)[3][Sizes<P>::ND]) {
+    constexpr int ND = Sizes<P>::ND;
+    const int ne = A.ne;
+    if (WD) {
+        double qv[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double m[3] = {(double)nq[k][0], (double)nq[k][1], (double)nq[k][2]};
+            to_vertices(m, qv[k]);
+        }
+        const double bv[3] = {A.bath_vrtx[e], A.bath_vrtx[(size_t)ne + e], A.bath_vrtx[2 * (size_t)ne + e]};
+        const double bmin = rmin(rmin(bv[0], bv[1]), bv[2]);
+        bool rewrite = false;
+        const bool wet_new = wet_dry_vertices(qv, bv, bmin, wet, A.h_o, A.h_thr, &rewrite);
+        if (rewrite)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                double m[3];
+                from_vertices(qv[k], m);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) nq[k][d] = (Real)m[d];
+#pragma unroll
+                for (int d = 3; d < ND; ++d) nq[k][d] = Real(0.0);
+            }
+        A.wet_out[e] = wet_new ? 1 : 0;
+    }
+    bool has_nan = false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            A.q_out[(size_t)(k * ND + d) * ne + e] = nq[k][d];
+            has_nan |= (nq[k][d] != nq[k][d]);
+        }
+    if (A.nan_check && has_nan) {
+        const unsigned long long key = (A.stage_key << 32) | (unsigned long long)(unsigned)e;
+#ifdef __HIP_DEVICE_COMPILE__
+        atomicMin(A.nan_key, key);
+#else
+        if (key < *A.nan_key) *A.nan_key = key;
+#endif
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// The fused stage of one element, reference arithmetic order (FP64 plain form).
+
+template <typename Real, int P, bool WD, bool CF>
+RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e) {
+    using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
     const bool wet = WD ? (A.wet_in[e] != 0) : true;
     const Real grav = A.g;
 
     Real q[3][ND];
+#pragma unroll
     for (int k = 0; k < 3; ++k)
+#pragma unroll
         for (int d = 0; d < ND; ++d) q[k][d] = A.q_s[(size_t)(k * ND + d) * ne + e];
     Real bath[ND];
+#pragma unroll
     for (int d = 0; d < ND; ++d) bath[d] = A.bath[(size_t)d * ne + e];
     const Real J00 = A.geom[e], J01 = A.geom[(size_t)ne + e], J10 = A.geom[2 * (size_t)ne + e],
                J11 = A.geom[3 * (size_t)ne + e], adet = A.geom[4 * (size_t)ne + e];
@@ -339,30 +401,36 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
     if (wet) {
         // ---- volume_kernel (rkdg_swe_proc_volume.hpp:8-35): rhs = Fx*Dx + Fy*Dy
         Real ax[3][ND], ay[3][ND];
+#pragma unroll
         for (int g = 0; g < NG; ++g) {
             Real qg[3];
+#pragma unroll
             for (int k = 0; k < 3; ++k) {
-                Real s = q[k][0] * T.phi[0][g];
-                for (int d = 1; d < ND; ++d) s = s + q[k][d] * T.phi[d][g];
+                Real s = q[k][0] * C::phi[0][g];
+#pragma unroll
+                for (int d = 1; d < ND; ++d) s = s + q[k][d] * C::phi[d][g];
                 qg[k] = s;
             }
-            Real bg = bath[0] * T.phi[0][g];
-            for (int d = 1; d < ND; ++d) bg = bg + bath[d] * T.phi[d][g];
+            Real bg = bath[0] * C::phi[0][g];
+#pragma unroll
+            for (int d = 1; d < ND; ++d) bg = bg + bath[d] * C::phi[d][g];
             const Real h = qg[0] + bg;
             const Real u = qg[1] / h, v = qg[2] / h;
             const Real uuh = u * qg[1], vvh = v * qg[2], uvh = u * qg[2];
             const Real pe = grav * (Real(0.5) * (qg[0] * qg[0]) + qg[0] * bg);
             const Real fx[3] = {qg[1], uuh + pe, uvh};
             const Real fy[3] = {qg[2], uvh, vvh + pe};
+#pragma unroll
             for (int d = 0; d < ND; ++d) {
                 Real dx = Real(0.0);
-                dx += T.idphi[0][g][d] * J00;
-                dx += T.idphi[1][g][d] * J10;
+                dx += C::idphi0[g][d] * J00;
+                dx += C::idphi1[g][d] * J10;
                 dx *= adet;
                 Real dy = Real(0.0);
-                dy += T.idphi[0][g][d] * J01;
-                dy += T.idphi[1][g][d] * J11;
+                dy += C::idphi0[g][d] * J01;
+                dy += C::idphi1[g][d] * J11;
                 dy *= adet;
+#pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     if (g == 0) {
                         ax[k][d] = fx[k] * dx;
@@ -374,51 +442,55 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
                 }
             }
         }
+#pragma unroll
         for (int k = 0; k < 3; ++k)
+#pragma unroll
             for (int d = 0; d < ND; ++d) rhs[k][d] = ax[k][d] + ay[k][d];
 
         // ---- source_kernel (rkdg_swe_proc_source.hpp:8-18, swe_source.hpp:8-97)
         Real ts[3][ND];
+#pragma unroll
         for (int g = 0; g < NG; ++g) {
             Real qg[3];
+#pragma unroll
             for (int k = 0; k < 3; ++k) {
-                Real s = q[k][0] * T.phi[0][g];
-                for (int d = 1; d < ND; ++d) s = s + q[k][d] * T.phi[d][g];
+                Real s = q[k][0] * C::phi[0][g];
+#pragma unroll
+                for (int d = 1; d < ND; ++d) s = s + q[k][d] * C::phi[d][g];
                 qg[k] = s;
             }
-            Real dbx, dby;
-            {
-                Real sx = Real(0), sy = Real(0);
-                for (int d = 0; d < ND; ++d) {
-                    Real ex = Real(0.0);
-                    ex += T.dphi[0][d][g] * J00;
-                    ex += T.dphi[1][d][g] * J10;
-                    Real ey = Real(0.0);
-                    ey += T.dphi[0][d][g] * J01;
-                    ey += T.dphi[1][d][g] * J11;
-                    if (d == 0) {
-                        sx = bath[0] * ex;
-                        sy = bath[0] * ey;
-                    } else {
-                        sx = sx + bath[d] * ex;
-                        sy = sy + bath[d] * ey;
-                    }
+            Real dbx = Real(0), dby = Real(0);
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                Real ex = Real(0.0);
+                ex += C::dphi0[d][g] * J00;
+                ex += C::dphi1[d][g] * J10;
+                Real ey = Real(0.0);
+                ey += C::dphi0[d][g] * J01;
+                ey += C::dphi1[d][g] * J11;
+                if (d == 0) {
+                    dbx = bath[0] * ex;
+                    dby = bath[0] * ey;
+                } else {
+                    dbx = dbx + bath[d] * ex;
+                    dby = dby + bath[d] * ey;
                 }
-                dbx = sx, dby = sy;
             }
             Real src[3] = {Real(0.0), grav * (dbx * qg[0]), grav * (dby * qg[0])};
-            if (A.fsrc) {
-                const Real x = (A.vxy[e] * T.psi[0][g] + A.vxy[(size_t)ne + e] * T.psi[1][g]) +
-                               A.vxy[2 * (size_t)ne + e] * T.psi[2][g];
-                const Real y = (A.vxy[3 * (size_t)ne + e] * T.psi[0][g] + A.vxy[4 * (size_t)ne + e] * T.psi[1][g]) +
-                               A.vxy[5 * (size_t)ne + e] * T.psi[2][g];
+            if (CF && A.fsrc) {
+                const Real x = (A.vxy[e] * C::psi[0][g] + A.vxy[(size_t)ne + e] * C::psi[1][g]) +
+                               A.vxy[2 * (size_t)ne + e] * C::psi[2][g];
+                const Real y = (A.vxy[3 * (size_t)ne + e] * C::psi[0][g] + A.vxy[4 * (size_t)ne + e] * C::psi[1][g]) +
+                               A.vxy[5 * (size_t)ne + e] * C::psi[2][g];
                 Real f[3];
                 problem_source<Real>(A.t, x, y, f);
+#pragma unroll
                 for (int k = 0; k < 3; ++k) src[k] = src[k] + f[k];
             }
             if (A.friction) {
-                Real bg = bath[0] * T.phi[0][g];
-                for (int d = 1; d < ND; ++d) bg = bg + bath[d] * T.phi[d][g];
+                Real bg = bath[0] * C::phi[0][g];
+#pragma unroll
+                for (int d = 1; d < ND; ++d) bg = bg + bath[d] * C::phi[d][g];
                 const Real h = qg[0] + bg;
                 const Real u = qg[1] / h, v = qg[2] / h;
                 Real Cf = A.cf;
@@ -432,59 +504,81 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
                 src[1] -= tau * qg[1];
                 src[2] -= tau * qg[2];
             }
+#pragma unroll
             for (int d = 0; d < ND; ++d) {
-                const Real w = T.ipf[g][d] * adet;
+                const Real w = C::ipf[g][d] * adet;
+#pragma unroll
                 for (int k = 0; k < 3; ++k) ts[k][d] = (g == 0) ? src[k] * w : ts[k][d] + src[k] * w;
             }
         }
+#pragma unroll
         for (int k = 0; k < 3; ++k)
+#pragma unroll
             for (int d = 0; d < ND; ++d) rhs[k][d] = rhs[k][d] + ts[k][d];
     } else {
+#pragma unroll
         for (int k = 0; k < 3; ++k)
+#pragma unroll
             for (int d = 0; d < ND; ++d) rhs[k][d] = Real(0.0);
     }
 
     // ---- faces: interface_kernel / boundary_kernel contributions, in reference order
     const uint32_t meta = A.meta[e];
+#pragma unroll 1
     for (int s = 0; s < 3; ++s) {
         const int nbv = A.nb[(size_t)s * ne + e];
         const int ed = A.edge[(size_t)s * ne + e];
         const int f = (meta >> (4 * s)) & 3;
         const int fn = (meta >> (4 * s + 2)) & 3;
+        const Real* pf = FT.phie + (size_t)f * ND * NE;
+        const bool interior = nbv >= 0;
+        const bool wet_nb = (WD && interior) ? (A.wet_in[nbv] != 0) : true;
+        if (interior ? !(wet || wet_nb) : !wet) continue;
         const Real nx = A.egeom[ed], ny = A.egeom[(size_t)A.nedge + ed], sJ = A.egeom[2 * (size_t)A.nedge + ed];
-        Real F[3][NE];
-        if (nbv >= 0) {
-            const bool wet_nb = WD ? (A.wet_in[nbv] != 0) : true;
-            if (!(wet || wet_nb)) continue;
-            const bool is_in = e < nbv;
-            Real qo[3][NE], bo[NE];
-            for (int g = 0; g < NE; ++g) {
-                for (int k = 0; k < 3; ++k) {
-                    Real acc = q[k][0] * T.phie[f][0][g];
-                    for (int d = 1; d < ND; ++d) acc = acc + q[k][d] * T.phie[f][d][g];
-                    qo[k][g] = acc;
-                }
-                Real acc = bath[0] * T.phie[f][0][g];
-                for (int d = 1; d < ND; ++d) acc = acc + bath[d] * T.phie[f][d][g];
-                bo[g] = acc;
+        Real qo[3][NE], bo[NE];
+#pragma unroll
+        for (int g = 0; g < NE; ++g) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Real acc = q[k][0] * pf[g];
+#pragma unroll
+                for (int d = 1; d < ND; ++d) acc = acc + q[k][d] * pf[d * NE + g];
+                qo[k][g] = acc;
             }
+            Real acc = bath[0] * pf[g];
+#pragma unroll
+            for (int d = 1; d < ND; ++d) acc = acc + bath[d] * pf[d * NE + g];
+            bo[g] = acc;
+        }
+        Real F[3][NE];
+        if (interior) {
+            const bool is_in = e < nbv;
+            const Real* pn = FT.phie + (size_t)fn * ND * NE;
             Real qn[3][NE], bn[NE];
             {
                 Real qnb[3][ND], bnb[ND];
+#pragma unroll
                 for (int k = 0; k < 3; ++k)
+#pragma unroll
                     for (int d = 0; d < ND; ++d) qnb[k][d] = A.q_s[(size_t)(k * ND + d) * ne + nbv];
+#pragma unroll
                 for (int d = 0; d < ND; ++d) bnb[d] = A.bath[(size_t)d * ne + nbv];
+#pragma unroll
                 for (int g = 0; g < NE; ++g) {
+#pragma unroll
                     for (int k = 0; k < 3; ++k) {
-                        Real acc = qnb[k][0] * T.phie[fn][0][g];
-                        for (int d = 1; d < ND; ++d) acc = acc + qnb[k][d] * T.phie[fn][d][g];
+                        Real acc = qnb[k][0] * pn[g];
+#pragma unroll
+                        for (int d = 1; d < ND; ++d) acc = acc + qnb[k][d] * pn[d * NE + g];
                         qn[k][g] = acc;
                     }
-                    Real acc = bnb[0] * T.phie[fn][0][g];
-                    for (int d = 1; d < ND; ++d) acc = acc + bnb[d] * T.phie[fn][d][g];
+                    Real acc = bnb[0] * pn[g];
+#pragma unroll
+                    for (int d = 1; d < ND; ++d) acc = acc + bnb[d] * pn[d * NE + g];
                     bn[g] = acc;
                 }
             }
+#pragma unroll
             for (int g = 0; g < NE; ++g) {
                 const int gm = NE - 1 - g;  // mirrored point on the other side
                 Real Fg[3];
@@ -500,19 +594,8 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
         } else {
-            if (!wet) continue;
             const int bc = -1 - nbv;
-            Real qo[3][NE], bo[NE];
-            for (int g = 0; g < NE; ++g) {
-                for (int k = 0; k < 3; ++k) {
-                    Real acc = q[k][0] * T.phie[f][0][g];
-                    for (int d = 1; d < ND; ++d) acc = acc + q[k][d] * T.phie[f][d][g];
-                    qo[k][g] = acc;
-                }
-                Real acc = bath[0] * T.phie[f][0][g];
-                for (int d = 1; d < ND; ++d) acc = acc + bath[d] * T.phie[f][d][g];
-                bo[g] = acc;
-            }
+#pragma unroll
             for (int g = 0; g < NE; ++g) {
                 const Real qi[3] = {qo[0][g], qo[1][g], qo[2][g]};
                 Real qe[3];
@@ -522,26 +605,20 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
                     const Real qt_ex = qi[1] * tx + qi[2] * ty;
                     qe[0] = qi[0], qe[1] = qn_ex * nx + qt_ex * tx, qe[2] = qn_ex * ny + qt_ex * ty;
                 } else if (bc == 1 || bc == 2) {  // Tide / Flow (rkdg_swe_bc_tide.hpp:70-96, rkdg_swe_bc_flow.hpp:72-101)
-                    const int fr = A.eforce[ed];
-                    Real acc = Real(0.0);
-                    for (int c = 0; c < A.ncon; ++c) {
-                        using std::cos;
-                        const Real amp = A.famp[((size_t)fr * A.ncon + c) * NE + g];
-                        const Real ph = A.fphase[((size_t)fr * A.ncon + c) * NE + g];
-                        acc += Real(A.ramp) * A.ffact[c] * amp *
-                               cos(A.freq[c] * Real(A.t) + (A.eqarg[c] - ph) * Real(kPI) / Real(180));
-                    }
+                    const Real acc = harmonic_forcing<Real, NE>(A, ed, g);
                     if (bc == 1)
                         qe[0] = acc, qe[1] = qi[1], qe[2] = qi[2];
                     else
                         qe[0] = qi[0], qe[1] = acc * nx, qe[2] = acc * ny;
-                } else {  // Function (rkdg_swe_bc_function.hpp:139-172): ic_q(t, x_gp)
-                    const Real x = (A.vxy[e] * T.psie[f][0][g] + A.vxy[(size_t)ne + e] * T.psie[f][1][g]) +
-                                   A.vxy[2 * (size_t)ne + e] * T.psie[f][2][g];
-                    const Real y =
-                        (A.vxy[3 * (size_t)ne + e] * T.psie[f][0][g] + A.vxy[4 * (size_t)ne + e] * T.psie[f][1][g]) +
-                        A.vxy[5 * (size_t)ne + e] * T.psie[f][2][g];
+                } else if (CF) {  // Function (rkdg_swe_bc_function.hpp:139-172): ic_q(t, x_gp)
+                    const Real* ps = FT.psie + (size_t)f * 3 * NE;
+                    const Real x = (A.vxy[e] * ps[g] + A.vxy[(size_t)ne + e] * ps[NE + g]) +
+                                   A.vxy[2 * (size_t)ne + e] * ps[2 * NE + g];
+                    const Real y = (A.vxy[3 * (size_t)ne + e] * ps[g] + A.vxy[4 * (size_t)ne + e] * ps[NE + g]) +
+                                   A.vxy[5 * (size_t)ne + e] * ps[2 * NE + g];
                     problem_ic<Real>(A.problem, A.pparam, grav, A.t, x, y, qe);
+                } else {
+                    qe[0] = qi[0], qe[1] = qi[1], qe[2] = qi[2];  // unreachable: function BCs launch the CF build
                 }
                 Real Fg[3];
                 llf(grav, qi, qe, bo[g], nx, ny, Fg);
@@ -549,21 +626,27 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
             }
         }
         // rhs -= F * (phi_e * (w sJ))   (IntegrationPhiIN/EX, interface.hpp:170-182)
+#pragma unroll
         for (int d = 0; d < ND; ++d) {
             Real acc[3];
+#pragma unroll
             for (int g = 0; g < NE; ++g) {
-                const Real w = T.phie[f][d][g] * (T.we[g] * sJ);
+                const Real w = pf[d * NE + g] * (C::we[0][g] * sJ);
+#pragma unroll
                 for (int k = 0; k < 3; ++k) acc[k] = (g == 0) ? F[k][g] * w : acc[k] + F[k][g] * w;
             }
+#pragma unroll
             for (int k = 0; k < 3; ++k) rhs[k][d] = rhs[k][d] - acc[k];
         }
     }
 
     // ---- M^-1 and the SSP-RK combination (element.hpp:500-505, explicit_ssp_rk_stepper.hpp:67-80)
     Real nq[3][ND];
+#pragma unroll
     for (int k = 0; k < 3; ++k)
+#pragma unroll
         for (int d = 0; d < ND; ++d) {
-            const Real sol = rhs[k][d] * (T.minv[d] / adet);
+            const Real sol = rhs[k][d] * (C::minv[0][d] / adet);
             if (A.sol_out) A.sol_out[(size_t)(k * ND + d) * ne + e] = sol;
             Real acc = Real(0.0);
             for (int i = 0; i < A.nterm; ++i) {
@@ -575,43 +658,312 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const Tables<Real, P
             acc = acc + (A.cur_a * q[k][d] + A.cur_c * sol);
             nq[k][d] = acc;
         }
+    finish_element<Real, P, WD>(A, e, wet, nq);
+}
 
-    // ---- wetting_drying_kernel on the new state (swe_post_wet_dry.hpp:5-145), FP64
-    bool wet_new = wet;
-    if (WD) {
-        double qv[3][3];
-        for (int k = 0; k < 3; ++k) {
-            const double m[3] = {(double)nq[k][0], (double)nq[k][1], (double)nq[k][2]};
-            to_vertices(m, qv[k]);
-        }
-        const double bv[3] = {A.bath_vrtx[e], A.bath_vrtx[(size_t)ne + e], A.bath_vrtx[2 * (size_t)ne + e]};
-        const double bmin = rmin(rmin(bv[0], bv[1]), bv[2]);
-        bool rewrite = false;
-        wet_new = wet_dry_vertices(qv, bv, bmin, wet, A.h_o, A.h_thr, &rewrite);
-        if (rewrite)
-            for (int k = 0; k < 3; ++k) {
-                double m[3];
-                from_vertices(qv[k], m);
-                for (int d = 0; d < 3; ++d) nq[k][d] = (Real)m[d];
-                for (int d = 3; d < ND; ++d) nq[k][d] = Real(0.0);
-            }
-        A.wet_out[e] = wet_new ? 1 : 0;
-    }
+// ---------------------------------------------------------------------------------------
+// FP32 bulk form: the same stage with fewer operations.  Fluxes are rotated to reference
+// coordinates once per point (G1 = J00 Fx + J01 Fy, G2 = J10 Fx + J11 Fy) instead of building the
+// per-element D matrices; volume, source and lifts accumulate unscaled and M^-1/|det J| is applied
+// once; divisions become reciprocals; faces run in local-face order so the own-face tables are
+// compile-time constants (the neighbour's face table comes from LDS).  Both neighbours still
+// evaluate an interface flux with identical operands (in side = lower element ID), so fluxes
+// stay exactly antisymmetric and mass is conserved to roundoff.
 
-    bool has_nan = false;
-    for (int k = 0; k < 3; ++k)
-        for (int d = 0; d < ND; ++d) {
-            A.q_out[(size_t)(k * ND + d) * ne + e] = nq[k][d];
-            has_nan |= (nq[k][d] != nq[k][d]);
-        }
-    if (A.nan_check && has_nan) {
-        const unsigned long long key = (A.stage_key << 32) | (unsigned long long)(unsigned)e;
 #ifdef __HIP_DEVICE_COMPILE__
-        atomicMin(A.nan_key, key);
+#define RKDG_RCPF(x) __builtin_amdgcn_rcpf(x)
+#define RKDG_SQRTF(x) __builtin_amdgcn_sqrtf(x)
+#define RKDG_CBRT_RCPF(x) __builtin_amdgcn_exp2f(-0.333333343f * __builtin_amdgcn_logf(x))
 #else
-        if (key < *A.nan_key) *A.nan_key = key;
+#define RKDG_RCPF(x) (1.0f / (x))
+#define RKDG_SQRTF(x) std::sqrt(x)
+#define RKDG_CBRT_RCPF(x) std::pow((x), -1.0f / 3.0f)
 #endif
+
+RKDG_HD inline void llf_fast(float grav, const float* qi, const float* qx, float bath, float nx, float ny, float* F) {
+    const float h_in = qi[0] + bath, h_ex = qx[0] + bath;
+    const float r_in = RKDG_RCPF(h_in), r_ex = RKDG_RCPF(h_ex);
+    const float u_in = qi[1] * r_in, v_in = qi[2] * r_in;
+    const float u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
+    const float un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
+    const float lam = rmax(fabsf(un_in) + RKDG_SQRTF(grav * h_in), fabsf(un_ex) + RKDG_SQRTF(grav * h_ex));
+    const float pe_in = grav * qi[0] * (0.5f * qi[0] + bath), pe_ex = grav * qx[0] * (0.5f * qx[0] + bath);
+    const float fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
+    const float fi1 = u_in * fi0 + pe_in * nx, fe1 = u_ex * fe0 + pe_ex * nx;
+    const float fi2 = v_in * fi0 + pe_in * ny, fe2 = v_ex * fe0 + pe_ex * ny;
+    F[0] = 0.5f * (fi0 + fe0 + lam * (qi[0] - qx[0]));
+    F[1] = 0.5f * (fi1 + fe1 + lam * (qi[1] - qx[1]));
+    F[2] = 0.5f * (fi2 + fe2 + lam * (qi[2] - qx[2]));
+}
+
+template <bool WD>
+RKDG_HD inline void interface_flux_fast(float grav, const float* qi, const float* qx, float bi, float bx, float nx,
+                                        float ny, bool wet_in, bool wet_ex, bool want_in, float* F) {
+    float Fi[3], Fe[3];
+    llf_fast(grav, qi, qx, bi, nx, ny, Fi);
+    Fe[0] = -Fi[0], Fe[1] = -Fi[1], Fe[2] = -Fi[2];
+    if (WD) {  // the wet/dry fixes of rkdg_swe_is_internal.hpp:47-90
+        if (Fi[0] > 1e-12f) {
+            if (!wet_in) {
+                Fi[0] = 0, Fi[1] = 0, Fi[2] = 0;
+                const float tx = ny, ty = -nx, mx = -nx, my = -ny;
+                const float qn_ex = -(qx[1] * mx + qx[2] * my), qt_ex = qx[1] * tx + qx[2] * ty;
+                const float qe[3] = {qx[0], qn_ex * mx + qt_ex * tx, qn_ex * my + qt_ex * ty};
+                llf_fast(grav, qx, qe, bx, mx, my, Fe);
+            } else if (!wet_ex) {
+                const float f0 = Fi[0];
+                llf_fast(0.0f, qx, qi, bx, -nx, -ny, Fe);
+                Fe[0] = -f0;
+            }
+        } else if (Fi[0] < -1e-12f) {
+            if (!wet_ex) {
+                Fe[0] = 0, Fe[1] = 0, Fe[2] = 0;
+                const float tx = -ny, ty = nx;
+                const float qn_ex = -(qi[1] * nx + qi[2] * ny), qt_ex = qi[1] * tx + qi[2] * ty;
+                const float qe[3] = {qi[0], qn_ex * nx + qt_ex * tx, qn_ex * ny + qt_ex * ty};
+                llf_fast(grav, qi, qe, bi, nx, ny, Fi);
+            } else if (!wet_in) {
+                const float fe0 = Fe[0];
+                llf_fast(0.0f, qi, qx, bi, nx, ny, Fi);
+                Fi[0] = -fe0;
+            }
+        }
     }
+    const float* src = want_in ? Fi : Fe;
+    F[0] = src[0], F[1] = src[1], F[2] = src[2];
+}
+
+// Exterior state of a tide, flow or function boundary point.  Tide/flow: rkdg_swe_bc_tide.hpp:70-96, rkdg_swe_bc_flow.hpp:72-101;
+// function: rkdg_swe_bc_function.hpp:139-172.
+template <typename Real, int P, bool CF>
+RKDG_HD inline void exterior_state(const StageArgs<Real>& A, int bc, int e, int ed, int f, int g, const Real* qi,
+                                   Real nx, Real ny, Real* qe) {
+    constexpr int NE = Sizes<P>::NE;
+    if (bc == 1 || bc == 2) {
+        const Real acc = harmonic_forcing<Real, NE>(A, ed, g);
+        if (bc == 1)
+            qe[0] = acc, qe[1] = qi[1], qe[2] = qi[2];
+        else
+            qe[0] = qi[0], qe[1] = acc * nx, qe[2] = acc * ny;
+    } else if (CF) {
+        using C = ConstTables<Real, P>;
+        const int ne = A.ne;
+        const Real x = A.vxy[e] * C::psie[f * 3][g] + A.vxy[(size_t)ne + e] * C::psie[f * 3 + 1][g] +
+                       A.vxy[2 * (size_t)ne + e] * C::psie[f * 3 + 2][g];
+        const Real y = A.vxy[3 * (size_t)ne + e] * C::psie[f * 3][g] + A.vxy[4 * (size_t)ne + e] * C::psie[f * 3 + 1][g] +
+                       A.vxy[5 * (size_t)ne + e] * C::psie[f * 3 + 2][g];
+        problem_ic<Real>(A.problem, A.pparam, A.g, A.t, x, y, qe);
+    } else {
+        qe[0] = qi[0], qe[1] = qi[1], qe[2] = qi[2];  // unreachable: function BCs launch the CF build
+    }
+}
+
+template <int P, bool WD, bool CF>
+RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* phie_lds, const float* phiew_lds,
+                                       int e) {
+    using C = ConstTables<float, P>;
+    constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
+    const int ne = A.ne;
+    const float grav = A.g;
+
+    // ---- every load this element needs, issued before any arithmetic
+    float q[3][ND], bath[ND];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) q[k][d] = A.q_s[(size_t)(k * ND + d) * ne + e];
+#pragma unroll
+    for (int d = 0; d < ND; ++d) bath[d] = A.bath[(size_t)d * ne + e];
+    const float J00 = A.geom[e], J01 = A.geom[(size_t)ne + e], J10 = A.geom[2 * (size_t)ne + e],
+                J11 = A.geom[3 * (size_t)ne + e], adet = A.geom[4 * (size_t)ne + e];
+    const uint32_t meta = A.meta[e];
+    int nbv[3], ed[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) nbv[f] = A.nb[(size_t)f * ne + e], ed[f] = A.edge[(size_t)f * ne + e];
+    float qn_m[3][3][ND], bn_m[3][ND];  // neighbour modes across each face
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        const int n = nbv[f] >= 0 ? nbv[f] : e;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) qn_m[f][k][d] = A.q_s[(size_t)(k * ND + d) * ne + n];
+            bn_m[f][d] = A.bath[(size_t)d * ne + n];
+        }
+    }
+    float nx[3], ny[3], sJ[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+        nx[f] = A.egeom[ed[f]], ny[f] = A.egeom[(size_t)A.nedge + ed[f]], sJ[f] = A.egeom[2 * (size_t)A.nedge + ed[f]];
+    const bool wet = WD ? (A.wet_in[e] != 0) : true;
+    bool wet_nb[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) wet_nb[f] = (WD && nbv[f] >= 0) ? (A.wet_in[nbv[f]] != 0) : true;
+    const float gmn2 = A.friction == 2 ? A.gmn2[e] : 0.0f;
+    const float rdet = RKDG_RCPF(adet);
+
+    // ---- volume and source: rhs (unscaled by |det J|) = sum_g [idphi . (J^-T F) + ipf . S]
+    float rhs[3][ND];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) rhs[k][d] = 0.0f;
+    if (wet) {
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            float qg[3] = {0.0f, 0.0f, 0.0f}, bg = 0.0f, bz1 = 0.0f, bz2 = 0.0f;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) qg[k] += q[k][d] * C::phi[d][g];
+                bg += bath[d] * C::phi[d][g];
+                bz1 += bath[d] * C::dphi0[d][g];
+                bz2 += bath[d] * C::dphi1[d][g];
+            }
+            const float h = qg[0] + bg, rh = RKDG_RCPF(h);
+            const float u = qg[1] * rh, v = qg[2] * rh;
+            const float pe = grav * qg[0] * (0.5f * qg[0] + bg);
+            const float uvh = u * qg[2];
+            const float fx[3] = {qg[1], u * qg[1] + pe, uvh};
+            const float fy[3] = {qg[2], uvh, v * qg[2] + pe};
+            float G1[3], G2[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) G1[k] = J00 * fx[k] + J01 * fy[k], G2[k] = J10 * fx[k] + J11 * fy[k];
+            const float gze = grav * qg[0];
+            float S[3] = {0.0f, gze * (bz1 * J00 + bz2 * J10), gze * (bz1 * J01 + bz2 * J11)};
+            if (CF && A.fsrc) {
+                const float x = A.vxy[e] * C::psi[0][g] + A.vxy[(size_t)ne + e] * C::psi[1][g] +
+                                A.vxy[2 * (size_t)ne + e] * C::psi[2][g];
+                const float y = A.vxy[3 * (size_t)ne + e] * C::psi[0][g] + A.vxy[4 * (size_t)ne + e] * C::psi[1][g] +
+                                A.vxy[5 * (size_t)ne + e] * C::psi[2][g];
+                float fsv[3];
+                problem_source<float>(A.t, x, y, fsv);
+                S[0] += fsv[0], S[1] += fsv[1], S[2] += fsv[2];
+            }
+            if (A.friction) {
+                float Cf = A.cf;
+                if (A.friction == 2) {
+                    Cf = gmn2 * RKDG_CBRT_RCPF(h);
+                    if (Cf < A.cf) Cf = A.cf;
+                }
+                const float tau = Cf * RKDG_SQRTF(u * u + v * v) * rh;
+                S[1] -= tau * qg[1];
+                S[2] -= tau * qg[2];
+            }
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+                    rhs[k][d] += C::idphi0[g][d] * G1[k] + C::idphi1[g][d] * G2[k] + C::ipf[g][d] * S[k];
+        }
+    }
+
+    // ---- interior faces (local face order; own tables compile-time, the neighbour's from LDS)
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        if (nbv[f] < 0 || !(wet || wet_nb[f])) continue;
+        const int fn = (meta >> (4 * f + 2)) & 3;
+        const float* pn = phie_lds + fn * ND * NE;
+        const bool is_in = e < nbv[f];
+        float F[3][NE];
+#pragma unroll
+        for (int g = 0; g < NE; ++g) {
+            const int gm = NE - 1 - g;
+            float qo[3] = {0.0f, 0.0f, 0.0f}, bo = 0.0f, qn[3] = {0.0f, 0.0f, 0.0f}, bn = 0.0f;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                const float w = C::phie[f * ND + d][g];
+                const float wn = pn[d * NE + gm];
+#pragma unroll
+                for (int k = 0; k < 3; ++k) qo[k] += q[k][d] * w, qn[k] += qn_m[f][k][d] * wn;
+                bo += bath[d] * w, bn += bn_m[f][d] * wn;
+            }
+            float Fg[3];
+            if (is_in)
+                interface_flux_fast<WD>(grav, qo, qn, bo, bn, nx[f], ny[f], wet, wet_nb[f], true, Fg);
+            else
+                interface_flux_fast<WD>(grav, qn, qo, bn, bo, nx[f], ny[f], wet_nb[f], wet, false, Fg);
+            F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
+        }
+        const float scale = sJ[f] * rdet;
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+            for (int g = 0; g < NE; ++g) {
+                const float w = C::phiew[f * ND + d][g];
+                a0 += F[0][g] * w, a1 += F[1][g] * w, a2 += F[2][g] * w;
+            }
+            rhs[0][d] -= scale * a0, rhs[1][d] -= scale * a1, rhs[2][d] -= scale * a2;
+        }
+    }
+
+    // ---- boundary faces (rolled loop, tables from LDS)
+    if (wet && ((nbv[0] | nbv[1] | nbv[2]) < 0)) {
+#pragma unroll 1
+        for (int f = 0; f < 3; ++f) {
+            if (nbv[f] >= 0) continue;
+            const int bc = -1 - nbv[f];
+            const float* pf = phie_lds + f * ND * NE;
+            const float* pw = phiew_lds + f * ND * NE;
+            const float fnx = f == 0 ? nx[0] : (f == 1 ? nx[1] : nx[2]);
+            const float fny = f == 0 ? ny[0] : (f == 1 ? ny[1] : ny[2]);
+            const float fsJ = f == 0 ? sJ[0] : (f == 1 ? sJ[1] : sJ[2]);
+            const int fed = f == 0 ? ed[0] : (f == 1 ? ed[1] : ed[2]);
+            float F[3][NE];
+#pragma unroll
+            for (int g = 0; g < NE; ++g) {
+                float qi[3] = {0.0f, 0.0f, 0.0f}, bo = 0.0f;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const float w = pf[d * NE + g];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) qi[k] += q[k][d] * w;
+                    bo += bath[d] * w;
+                }
+                float qe[3];
+                if (bc == 0) {
+                    const float tx = -fny, ty = fnx;
+                    const float qn_ex = -(qi[1] * fnx + qi[2] * fny), qt_ex = qi[1] * tx + qi[2] * ty;
+                    qe[0] = qi[0], qe[1] = qn_ex * fnx + qt_ex * tx, qe[2] = qn_ex * fny + qt_ex * ty;
+                } else {
+                    exterior_state<float, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe);
+                }
+                float Fg[3];
+                llf_fast(grav, qi, qe, bo, fnx, fny, Fg);
+                F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
+            }
+            const float scale = fsJ * rdet;
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+                for (int g = 0; g < NE; ++g) {
+                    const float w = pw[d * NE + g];
+                    a0 += F[0][g] * w, a1 += F[1][g] * w, a2 += F[2][g] * w;
+                }
+                rhs[0][d] -= scale * a0, rhs[1][d] -= scale * a1, rhs[2][d] -= scale * a2;
+            }
+        }
+    }
+
+    float nq[3][ND];
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int d = 0; d < ND; ++d) {
+            const float sol = rhs[k][d] * C::minv[0][d];
+            if (A.sol_out) A.sol_out[(size_t)(k * ND + d) * ne + e] = sol;
+            float acc = A.cur_a * q[k][d] + A.cur_c * sol;
+            for (int i = 0; i < A.nterm; ++i) {
+                const size_t ix = (size_t)(k * ND + d) * ne + e;
+                float term = A.term_a[i] * A.term_q[i][ix];
+                if (A.term_sol[i]) term += A.term_c[i] * A.term_sol[i][ix];
+                acc += term;
+            }
+            nq[k][d] = acc;
+        }
+    finish_element<float, P, WD>(A, e, wet, nq);
 }
 
 }  // namespace rkdg

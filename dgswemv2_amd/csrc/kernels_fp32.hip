@@ -1,0 +1,8 @@
+// FP32 bulk form: built with FP contraction (fused multiply-adds); its error against the FP64
+// oracle is measured, not bit-matched.
+#include "kernels.inc"
+
+namespace rkdg {
+template hipError_t launch_stage<float>(int, bool, bool, const StageArgs<float>&, hipStream_t, int);
+template hipError_t launch_limiter<float>(int, const LimiterArgs<float>&, hipStream_t, int);
+}  // namespace rkdg

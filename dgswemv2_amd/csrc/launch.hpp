@@ -9,9 +9,9 @@
 namespace rkdg {
 
 template <typename Real>
-hipError_t launch_stage(int p, bool wd, const StageArgs<Real>& A, const void* tables, hipStream_t s, int block);
+hipError_t launch_stage(int p, bool wd, bool cf, const StageArgs<Real>& A, hipStream_t s, int block);
 
 template <typename Real>
-hipError_t launch_limiter(int p, const LimiterArgs<Real>& L, const void* tables, hipStream_t s, int block);
+hipError_t launch_limiter(int p, const LimiterArgs<Real>& L, hipStream_t s, int block);
 
 }  // namespace rkdg

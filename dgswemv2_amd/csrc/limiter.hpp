@@ -54,7 +54,7 @@ RKDG_HD inline void load_linear(const LimiterArgs<Real>& L, int e, double ql[3][
 }
 
 template <typename Real, int P>
-RKDG_HD inline void sl_prepare_element(const LimiterArgs<Real>& L, const Tables<Real, P>& T, int e) {
+RKDG_HD inline void sl_prepare_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
     constexpr int ND = Sizes<P>::ND, NE = Sizes<P>::NE;
     const int ne = L.ne;
     const bool wet = L.wet[e] != 0;
@@ -80,17 +80,17 @@ RKDG_HD inline void sl_prepare_element(const LimiterArgs<Real>& L, const Tables<
         const double onx = is_in ? nx : -nx, ony = is_in ? ny : -ny;
         double ze_o[NE], qx_o[NE], h_o[NE], h_n[NE];
         for (int g = 0; g < NE; ++g) {
-            double z = (double)L.q[(size_t)(0 * ND) * ne + e] * (double)T.phie[f][0][g];
-            double x = (double)L.q[(size_t)(1 * ND) * ne + e] * (double)T.phie[f][0][g];
-            double b = (double)L.bath[e] * (double)T.phie[f][0][g];
-            double zn = (double)L.q[(size_t)(0 * ND) * ne + nbv] * (double)T.phie[fn][0][g];
-            double bn = (double)L.bath[nbv] * (double)T.phie[fn][0][g];
+            double z = (double)L.q[(size_t)(0 * ND) * ne + e] * (double)FT.phie[((f) * ND + (0)) * NE + (g)];
+            double x = (double)L.q[(size_t)(1 * ND) * ne + e] * (double)FT.phie[((f) * ND + (0)) * NE + (g)];
+            double b = (double)L.bath[e] * (double)FT.phie[((f) * ND + (0)) * NE + (g)];
+            double zn = (double)L.q[(size_t)(0 * ND) * ne + nbv] * (double)FT.phie[((fn) * ND + (0)) * NE + (g)];
+            double bn = (double)L.bath[nbv] * (double)FT.phie[((fn) * ND + (0)) * NE + (g)];
             for (int d = 1; d < ND; ++d) {
-                z = z + (double)L.q[(size_t)(0 * ND + d) * ne + e] * (double)T.phie[f][d][g];
-                x = x + (double)L.q[(size_t)(1 * ND + d) * ne + e] * (double)T.phie[f][d][g];
-                b = b + (double)L.bath[(size_t)d * ne + e] * (double)T.phie[f][d][g];
-                zn = zn + (double)L.q[(size_t)(0 * ND + d) * ne + nbv] * (double)T.phie[fn][d][g];
-                bn = bn + (double)L.bath[(size_t)d * ne + nbv] * (double)T.phie[fn][d][g];
+                z = z + (double)L.q[(size_t)(0 * ND + d) * ne + e] * (double)FT.phie[((f) * ND + (d)) * NE + (g)];
+                x = x + (double)L.q[(size_t)(1 * ND + d) * ne + e] * (double)FT.phie[((f) * ND + (d)) * NE + (g)];
+                b = b + (double)L.bath[(size_t)d * ne + e] * (double)FT.phie[((f) * ND + (d)) * NE + (g)];
+                zn = zn + (double)L.q[(size_t)(0 * ND + d) * ne + nbv] * (double)FT.phie[((fn) * ND + (d)) * NE + (g)];
+                bn = bn + (double)L.bath[(size_t)d * ne + nbv] * (double)FT.phie[((fn) * ND + (d)) * NE + (g)];
             }
             ze_o[g] = z, qx_o[g] = x;
             h_o[g] = z + b;
@@ -100,7 +100,7 @@ RKDG_HD inline void sl_prepare_element(const LimiterArgs<Real>& L, const Tables<
         double integ = 0.0;
         for (int g = 0; g < NE; ++g) {
             const double qn = onx * ze_o[g] + ony * qx_o[g];
-            const double w = (double)T.we[g] * sJ;
+            const double w = (double)ConstTables<Real, P>::we[0][g] * sJ;
             integ = (g == 0) ? qn * w : integ + qn * w;
         }
         if (integ < 0.0) {
@@ -108,7 +108,7 @@ RKDG_HD inline void sl_prepare_element(const LimiterArgs<Real>& L, const Tables<
             for (int g = 0; g < NE; ++g) {
                 double hd = h_o[g];
                 hd -= h_n[NE - 1 - g];
-                const double w = (double)T.we[g] * sJ;
+                const double w = (double)ConstTables<Real, P>::we[0][g] * sJ;
                 hi = (g == 0) ? hd * w : hi + hd * w;
             }
             perimeter += L.len[(size_t)f * ne + e];
@@ -143,7 +143,7 @@ RKDG_HD inline void mv3(const double A[3][3], const double* x, double* y) {
 }
 
 template <typename Real, int P>
-RKDG_HD inline void sl_limit_element(const LimiterArgs<Real>& L, const Tables<Real, P>& T, int e) {
+RKDG_HD inline void sl_limit_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
     constexpr int ND = Sizes<P>::ND;
     const int ne = L.ne;
     const bool wet = L.wet[e] != 0;

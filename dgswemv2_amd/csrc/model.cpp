@@ -117,7 +117,8 @@ std::vector<double> shape_psi(const std::vector<std::array<double, 2>>& pts) {
 
 }  // namespace
 
-std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M) {
+std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M, bool reference_face_order) {
+    M.reference_face_order = reference_face_order;
     if (md.n_elements <= 0 || md.n_nodes <= 0 || !md.node_xyz || !md.elt_nodes) return "empty mesh";
     if (prm.p < 1 || prm.p > 2) return "polynomial order outside the device-built set {1, 2}";
     if (!M.stepper.init(prm.nstages, prm.order, prm.dt, prm.ramp_duration)) return "invalid Runge-Kutta method";
@@ -260,6 +261,8 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
         M.eforce.push_back(fr);
     }
     M.nedge = (int)eg[0].size();
+    M.closed_forms = prm.function_source != 0;
+    for (auto& b : bedges) M.closed_forms |= (b[0] == RKDG_BC_FUNCTION);
     M.egeom.resize(3 * (size_t)M.nedge);
     for (int c = 0; c < 3; ++c) std::copy(eg[c].begin(), eg[c].end(), M.egeom.begin() + (size_t)c * M.nedge);
 
@@ -278,7 +281,7 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
                 key[k] = {1, btype[3 * e + k], k};
             M.neighbor[(size_t)k * ne + e] = nbr[3 * e + k];
         }
-        std::sort(key.begin(), key.end());
+        if (reference_face_order) std::sort(key.begin(), key.end());
         uint32_t meta = 0;
         for (int s = 0; s < 3; ++s) {
             const int k = key[s][2];

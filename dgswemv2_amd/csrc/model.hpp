@@ -37,6 +37,8 @@ struct Model {
     std::vector<double> vxy;        // [6][ne] x0 x1 x2 y0 y1 y2 (after the CCW fix)
     std::vector<double> bath_vrtx;  // [3][ne]
     std::vector<double> gmn2;       // [ne]
+    bool reference_face_order = true;
+    bool closed_forms = false;      // manufactured source or function boundaries present
     std::vector<int> nb;            // [3][ne] per slot: neighbour, or -1-bc
     std::vector<int> edge;          // [3][ne] per slot: edge record
     std::vector<uint32_t> meta;     // [ne] own/neighbour local face per slot
@@ -65,7 +67,9 @@ struct Model {
 };
 
 // Builds the model; returns an empty string or an error message.
-std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M);
+// reference_face_order: face slots in the reference's rhs accumulation order (the FP64 form);
+// otherwise slots are the local faces 0, 1, 2 (the FP32 form).
+std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M, bool reference_face_order = true);
 
 // L2 residual against the problem's true solution (element.hpp:547-592), FP64, host.
 double l2_residual(const Model& M, const std::vector<double>& q, double t);
