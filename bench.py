@@ -72,6 +72,20 @@ def cpu_baseline(budget_s=12.0):
                       f"{el:.1f} s, FP64 oracle with OpenMP over {threads} threads (coloured interface loop)"}
 
 
+def copy_probe(mib):
+    """Five plain device-to-device copies of a known size: the HBM counters of this kernel
+    (FETCH_SIZE, WRITE_SIZE per dispatch) calibrate the counters read on the stage kernel."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {mib} MiB x 5 ({n * 4} bytes read and written per copy)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,7 +93,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=float, default=1.0, help="mesh scale (1.0 = the 1M-triangle workload)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--form", choices=("both", "fp64_plain", "fp32_bulk"), default="both",
+                    help="time one form only (short profiled runs); the JSON line needs both")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run 5 device copies of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine's multi-GPU sharding is "
               "described in DESIGN.md and not built", file=sys.stderr)
@@ -96,6 +117,8 @@ def main():
     forms = {}
     qs = {}
     for name, prec in (("fp64_plain", RKDG_FP64_PLAIN), ("fp32_bulk", RKDG_FP32_BULK)):
+        if args.form not in ("both", name):
+            continue
         log("timing", name, "warmup", args.warmup, "steps", args.steps)
         wall, rep, q = time_form(mesh, prm, prec, args.steps, args.warmup)
         qs[name] = q
@@ -111,6 +134,9 @@ def main():
             "roofline_frac": achieved / HBM_PEAK_GBS,
         }
         log(name, json.dumps(forms[name]))
+    if args.form != "both":
+        log("single form timed; no JSON line")
+        return
     # FP32-bulk error against the FP64 form after the same steps (same start state)
     rel = []
     for k in range(3):

@@ -42,7 +42,7 @@ template <typename Real>
 struct Engine final : EngineBase {
     rkdg::Model& M;
     hipStream_t stream;
-    int block = 256;
+    int block = 128;  // work-items per block (measured best for the stage kernels on 1M elements)
     std::vector<void*> allocs;
     std::vector<Real*> q;      // reference state slots 0..nstages
     std::vector<Real*> sol;    // per stage, only where a later stage reads it
@@ -64,7 +64,7 @@ struct Engine final : EngineBase {
 
     Engine(rkdg::Model& m, hipStream_t s) : M(m), stream(s) {
         if (const char* b = std::getenv("RKDG_BLOCK")) block = std::atoi(b);  // tuning experiments
-        if (block < 64 || block > 256 || block % 64) block = 256;
+        if (block < 64 || block > 256 || block % 64) block = 128;
     }
     ~Engine() override {
         for (auto& pr : stage_ev) hipEventDestroy(pr.first), hipEventDestroy(pr.second);

@@ -86,6 +86,45 @@ struct StageArgs {
 };
 
 // ---------------------------------------------------------------------------------------
+// Element-fastest arrays: entry i of column c sits at base[c * n + i].  On the GPU a column
+// access is a buffer instruction whose column offset lives in a scalar register, so a gather
+// costs one instruction and no per-lane 64-bit address arithmetic.  Byte offsets stay below
+// 2^32 (the model builder rejects larger meshes).
+
+#ifdef __HIP_DEVICE_COMPILE__
+RKDG_HD inline __amdgpu_buffer_rsrc_t rkdg_rsrc(const void* base) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, -1, 0x00020000);
+}
+#endif
+
+template <typename T>
+RKDG_HD inline T ldc(const T* base, int n, int c, int i) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const unsigned soff = (unsigned)c * (unsigned)n * (unsigned)sizeof(T);
+    if constexpr (sizeof(T) == 4)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(rkdg_rsrc(base), i * 4, soff, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(rkdg_rsrc(base), i * 8, soff, 0));
+#else
+    return base[(size_t)c * n + i];
+#endif
+}
+
+template <typename T>
+RKDG_HD inline void stc(T* base, int n, int c, int i, T v) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const unsigned soff = (unsigned)c * (unsigned)n * (unsigned)sizeof(T);
+    if constexpr (sizeof(T) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rkdg_rsrc(base), i * 4, soff, 0);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                              rkdg_rsrc(base), i * 8, soff, 0);
+#else
+    base[(size_t)c * n + i] = v;
+#endif
+}
+
+// ---------------------------------------------------------------------------------------
 // Closed-form problem functions (examples/swe_manufactured_solution/*.hpp,
 // examples/swe_parabolic_bowl/parabolic_bowl_initial_condition_functions.hpp:7-28).
 
@@ -341,7 +380,7 @@ RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Re
             const double m[3] = {(double)nq[k][0], (double)nq[k][1], (double)nq[k][2]};
             to_vertices(m, qv[k]);
         }
-        const double bv[3] = {A.bath_vrtx[e], A.bath_vrtx[(size_t)ne + e], A.bath_vrtx[2 * (size_t)ne + e]};
+        const double bv[3] = {ldc(A.bath_vrtx, ne, 0, e), ldc(A.bath_vrtx, ne, 1, e), ldc(A.bath_vrtx, ne, 2, e)};
         const double bmin = rmin(rmin(bv[0], bv[1]), bv[2]);
         bool rewrite = false;
         const bool wet_new = wet_dry_vertices(qv, bv, bmin, wet, A.h_o, A.h_thr, &rewrite);
@@ -362,7 +401,7 @@ RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Re
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            A.q_out[(size_t)(k * ND + d) * ne + e] = nq[k][d];
+            stc(A.q_out, ne, k * ND + d, e, nq[k][d]);
             has_nan |= (nq[k][d] != nq[k][d]);
         }
     if (A.nan_check && has_nan) {
@@ -390,12 +429,12 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) q[k][d] = A.q_s[(size_t)(k * ND + d) * ne + e];
+        for (int d = 0; d < ND; ++d) q[k][d] = ldc(A.q_s, ne, k * ND + d, e);
     Real bath[ND];
 #pragma unroll
-    for (int d = 0; d < ND; ++d) bath[d] = A.bath[(size_t)d * ne + e];
-    const Real J00 = A.geom[e], J01 = A.geom[(size_t)ne + e], J10 = A.geom[2 * (size_t)ne + e],
-               J11 = A.geom[3 * (size_t)ne + e], adet = A.geom[4 * (size_t)ne + e];
+    for (int d = 0; d < ND; ++d) bath[d] = ldc(A.bath, ne, d, e);
+    const Real J00 = ldc(A.geom, ne, 0, e), J01 = ldc(A.geom, ne, 1, e), J10 = ldc(A.geom, ne, 2, e),
+               J11 = ldc(A.geom, ne, 3, e), adet = ldc(A.geom, ne, 4, e);
 
     Real rhs[3][ND];
     if (wet) {
@@ -496,7 +535,7 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
                 Real Cf = A.cf;
                 if (A.friction == 2) {
                     using std::pow;
-                    Cf = A.gmn2[e] / pow(h, Real(1.0 / 3.0));
+                    Cf = ldc(A.gmn2, ne, 0, e) / pow(h, Real(1.0 / 3.0));
                     if (Cf < A.cf) Cf = A.cf;
                 }
                 using std::hypot;
@@ -523,18 +562,18 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
     }
 
     // ---- faces: interface_kernel / boundary_kernel contributions, in reference order
-    const uint32_t meta = A.meta[e];
+    const uint32_t meta = ldc(A.meta, ne, 0, e);
 #pragma unroll 1
     for (int s = 0; s < 3; ++s) {
-        const int nbv = A.nb[(size_t)s * ne + e];
-        const int ed = A.edge[(size_t)s * ne + e];
+        const int nbv = ldc(A.nb, ne, s, e);
+        const int ed = ldc(A.edge, ne, s, e);
         const int f = (meta >> (4 * s)) & 3;
         const int fn = (meta >> (4 * s + 2)) & 3;
         const Real* pf = FT.phie + (size_t)f * ND * NE;
         const bool interior = nbv >= 0;
         const bool wet_nb = (WD && interior) ? (A.wet_in[nbv] != 0) : true;
         if (interior ? !(wet || wet_nb) : !wet) continue;
-        const Real nx = A.egeom[ed], ny = A.egeom[(size_t)A.nedge + ed], sJ = A.egeom[2 * (size_t)A.nedge + ed];
+        const Real nx = ldc(A.egeom, A.nedge, 0, ed), ny = ldc(A.egeom, A.nedge, 1, ed), sJ = ldc(A.egeom, A.nedge, 2, ed);
         Real qo[3][NE], bo[NE];
 #pragma unroll
         for (int g = 0; g < NE; ++g) {
@@ -560,9 +599,9 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
 #pragma unroll
                 for (int k = 0; k < 3; ++k)
 #pragma unroll
-                    for (int d = 0; d < ND; ++d) qnb[k][d] = A.q_s[(size_t)(k * ND + d) * ne + nbv];
+                    for (int d = 0; d < ND; ++d) qnb[k][d] = ldc(A.q_s, ne, k * ND + d, nbv);
 #pragma unroll
-                for (int d = 0; d < ND; ++d) bnb[d] = A.bath[(size_t)d * ne + nbv];
+                for (int d = 0; d < ND; ++d) bnb[d] = ldc(A.bath, ne, d, nbv);
 #pragma unroll
                 for (int g = 0; g < NE; ++g) {
 #pragma unroll
@@ -647,12 +686,11 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const Real sol = rhs[k][d] * (C::minv[0][d] / adet);
-            if (A.sol_out) A.sol_out[(size_t)(k * ND + d) * ne + e] = sol;
+            if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
             Real acc = Real(0.0);
             for (int i = 0; i < A.nterm; ++i) {
-                const size_t ix = (size_t)(k * ND + d) * ne + e;
-                Real term = A.term_a[i] * A.term_q[i][ix];
-                if (A.term_sol[i]) term = term + A.term_c[i] * A.term_sol[i][ix];
+                Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                if (A.term_sol[i]) term = term + A.term_c[i] * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc = acc + term;
             }
             acc = acc + (A.cur_a * q[k][d] + A.cur_c * sol);
@@ -696,13 +734,36 @@ RKDG_HD inline void llf_fast(float grav, const float* qi, const float* qx, float
     F[2] = 0.5f * (fi2 + fe2 + lam * (qi[2] - qx[2]));
 }
 
+// The same LLF flux with the pressures of both states given relative to a hydrostatic
+// reference r: pi = pe_in - r, px = pe_ex - r.  The mass component and lambda are the plain
+// ones; the momentum components come out as F - r n.
+RKDG_HD inline void llf_rel(float grav, const float* qi, const float* qx, float bath, float pi, float px, float nx,
+                            float ny, float* F) {
+    const float h_in = qi[0] + bath, h_ex = qx[0] + bath;
+    const float r_in = RKDG_RCPF(h_in), r_ex = RKDG_RCPF(h_ex);
+    const float u_in = qi[1] * r_in, v_in = qi[2] * r_in;
+    const float u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
+    const float un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
+    const float lam = rmax(fabsf(un_in) + RKDG_SQRTF(grav * h_in), fabsf(un_ex) + RKDG_SQRTF(grav * h_ex));
+    const float fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
+    F[0] = 0.5f * (fi0 + fe0 + lam * (qi[0] - qx[0]));
+    F[1] = 0.5f * (u_in * fi0 + u_ex * fe0 + (pi + px) * nx + lam * (qi[1] - qx[1]));
+    F[2] = 0.5f * (v_in * fi0 + v_ex * fe0 + (pi + px) * ny + lam * (qi[2] - qx[2]));
+}
+
+// Interface flux of one side in the relative-pressure form: pi/px as for llf_rel, r the
+// caller's hydrostatic reference at this point.  Returns the caller's flux (w.r.t. its outward
+// normal) with momentum relative to r.  The wet/dry fixes of rkdg_swe_is_internal.hpp:47-90
+// (rare) are applied to the plain flux, rebuilt as F_rel + r n.
 template <bool WD>
-RKDG_HD inline void interface_flux_fast(float grav, const float* qi, const float* qx, float bi, float bx, float nx,
-                                        float ny, bool wet_in, bool wet_ex, bool want_in, float* F) {
-    float Fi[3], Fe[3];
-    llf_fast(grav, qi, qx, bi, nx, ny, Fi);
-    Fe[0] = -Fi[0], Fe[1] = -Fi[1], Fe[2] = -Fi[2];
-    if (WD) {  // the wet/dry fixes of rkdg_swe_is_internal.hpp:47-90
+RKDG_HD inline void interface_flux_fast(float grav, const float* qi, const float* qx, float pi, float px, float bi,
+                                        float bx, float nx, float ny, bool wet_in, bool wet_ex, bool want_in, float r,
+                                        float* F) {
+    float Fi[3];
+    llf_rel(grav, qi, qx, bi, pi, px, nx, ny, Fi);
+    if (WD && ((Fi[0] > 1e-12f && !(wet_in && wet_ex)) || (Fi[0] < -1e-12f && !(wet_in && wet_ex)))) {
+        Fi[1] += r * nx, Fi[2] += r * ny;
+        float Fe[3] = {-Fi[0], -Fi[1], -Fi[2]};
         if (Fi[0] > 1e-12f) {
             if (!wet_in) {
                 Fi[0] = 0, Fi[1] = 0, Fi[2] = 0;
@@ -710,27 +771,31 @@ RKDG_HD inline void interface_flux_fast(float grav, const float* qi, const float
                 const float qn_ex = -(qx[1] * mx + qx[2] * my), qt_ex = qx[1] * tx + qx[2] * ty;
                 const float qe[3] = {qx[0], qn_ex * mx + qt_ex * tx, qn_ex * my + qt_ex * ty};
                 llf_fast(grav, qx, qe, bx, mx, my, Fe);
-            } else if (!wet_ex) {
+            } else {  // !wet_ex
                 const float f0 = Fi[0];
                 llf_fast(0.0f, qx, qi, bx, -nx, -ny, Fe);
                 Fe[0] = -f0;
             }
-        } else if (Fi[0] < -1e-12f) {
+        } else {
             if (!wet_ex) {
                 Fe[0] = 0, Fe[1] = 0, Fe[2] = 0;
                 const float tx = -ny, ty = nx;
                 const float qn_ex = -(qi[1] * nx + qi[2] * ny), qt_ex = qi[1] * tx + qi[2] * ty;
                 const float qe[3] = {qi[0], qn_ex * nx + qt_ex * tx, qn_ex * ny + qt_ex * ty};
                 llf_fast(grav, qi, qe, bi, nx, ny, Fi);
-            } else if (!wet_in) {
+            } else {  // !wet_in
                 const float fe0 = Fe[0];
                 llf_fast(0.0f, qi, qx, bi, nx, ny, Fi);
                 Fi[0] = -fe0;
             }
         }
+        const float s = want_in ? 1.0f : -1.0f;
+        const float* src = want_in ? Fi : Fe;
+        F[0] = src[0], F[1] = src[1] - r * s * nx, F[2] = src[2] - r * s * ny;
+        return;
     }
-    const float* src = want_in ? Fi : Fe;
-    F[0] = src[0], F[1] = src[1], F[2] = src[2];
+    const float s = want_in ? 1.0f : -1.0f;
+    F[0] = s * Fi[0], F[1] = s * Fi[1], F[2] = s * Fi[2];
 }
 
 // Exterior state of a tide, flow or function boundary point.  Tide/flow: rkdg_swe_bc_tide.hpp:70-96, rkdg_swe_bc_flow.hpp:72-101;
@@ -758,6 +823,10 @@ RKDG_HD inline void exterior_state(const StageArgs<Real>& A, int bc, int e, int 
     }
 }
 
+// r + a * c for a compile-time table entry c; entries that are exactly zero (derivatives of the
+// constant mode, basis functions vanishing at a point) fold away at compile time.
+RKDG_HD inline float fmac_tab(float a, float c, float r) { return c != 0.0f ? fmaf(a, c, r) : r; }
+
 template <int P, bool WD, bool CF>
 RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* phie_lds, const float* phiew_lds,
                                        int e) {
@@ -771,36 +840,48 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) q[k][d] = A.q_s[(size_t)(k * ND + d) * ne + e];
+        for (int d = 0; d < ND; ++d) q[k][d] = ldc(A.q_s, ne, k * ND + d, e);
 #pragma unroll
-    for (int d = 0; d < ND; ++d) bath[d] = A.bath[(size_t)d * ne + e];
-    const float J00 = A.geom[e], J01 = A.geom[(size_t)ne + e], J10 = A.geom[2 * (size_t)ne + e],
-                J11 = A.geom[3 * (size_t)ne + e], adet = A.geom[4 * (size_t)ne + e];
-    const uint32_t meta = A.meta[e];
+    for (int d = 0; d < ND; ++d) bath[d] = ldc(A.bath, ne, d, e);
+    const float J00 = ldc(A.geom, ne, 0, e), J01 = ldc(A.geom, ne, 1, e), J10 = ldc(A.geom, ne, 2, e),
+                J11 = ldc(A.geom, ne, 3, e), adet = ldc(A.geom, ne, 4, e);
+    const uint32_t meta = ldc(A.meta, ne, 0, e);
     int nbv[3], ed[3];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) nbv[f] = A.nb[(size_t)f * ne + e], ed[f] = A.edge[(size_t)f * ne + e];
-    float qn_m[3][3][ND], bn_m[3][ND];  // neighbour modes across each face
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        const int n = nbv[f] >= 0 ? nbv[f] : e;
+    for (int f = 0; f < 3; ++f) nbv[f] = ldc(A.nb, ne, f, e), ed[f] = ldc(A.edge, ne, f, e);
+    // Neighbour modes are loaded one face ahead: face 0's before the volume integral, face f+1's
+    // while face f computes, so two faces' worth of registers are live at most.
+    auto load_nb = [&](int f, float (&qm)[3][ND], float (&bm)[ND]) {
+        const int n = nbv[f] >= 0 ? nbv[f] : e;  // boundary faces read their own (cached) modes
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
 #pragma unroll
-            for (int k = 0; k < 3; ++k) qn_m[f][k][d] = A.q_s[(size_t)(k * ND + d) * ne + n];
-            bn_m[f][d] = A.bath[(size_t)d * ne + n];
+            for (int k = 0; k < 3; ++k) qm[k][d] = ldc(A.q_s, ne, k * ND + d, n);
+            bm[d] = ldc(A.bath, ne, d, n);
         }
-    }
+    };
+    float qn_m[3][ND], bn_m[ND];
+    load_nb(0, qn_m, bn_m);
     float nx[3], ny[3], sJ[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f)
-        nx[f] = A.egeom[ed[f]], ny[f] = A.egeom[(size_t)A.nedge + ed[f]], sJ[f] = A.egeom[2 * (size_t)A.nedge + ed[f]];
+        nx[f] = ldc(A.egeom, A.nedge, 0, ed[f]), ny[f] = ldc(A.egeom, A.nedge, 1, ed[f]), sJ[f] = ldc(A.egeom, A.nedge, 2, ed[f]);
     const bool wet = WD ? (A.wet_in[e] != 0) : true;
     bool wet_nb[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) wet_nb[f] = (WD && nbv[f] >= 0) ? (A.wet_in[nbv[f]] != 0) : true;
-    const float gmn2 = A.friction == 2 ? A.gmn2[e] : 0.0f;
+    const float gmn2 = A.friction == 2 ? ldc(A.gmn2, ne, 0, e) : 0.0f;
     const float rdet = RKDG_RCPF(adet);
+
+    // ---- hydrostatic reference.  A wet element works with its momentum pressure relative to
+    // g zr (zr/2 + b), zr = its mean surface elevation.  With the quadratures exact for these
+    // degrees, volume + source of that part equal a face integral of g zr (zr/2 + b_own) n, so
+    // the volume uses pe - g zr (zr/2 + b) = g z' (h - z'/2) with z' = ze - zr, the source
+    // g z' grad b, and every face subtracts g zr (zr/2 + b_own) n.  Same discretisation; the
+    // large canceling terms (g ze b with b up to kilometres) never form in FP32.
+    constexpr float PHI0 = C::phi[0][0];  // the constant mode's value
+    const float zc = q[0][0] * PHI0;      // mean surface elevation (mode 0)
+    const float zr = wet ? zc : 0.0f;     // dry elements keep the plain form (no volume term)
 
     // ---- volume and source: rhs (unscaled by |det J|) = sum_g [idphi . (J^-T F) + ipf . S]
     float rhs[3][ND];
@@ -811,26 +892,27 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
     if (wet) {
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            float qg[3] = {0.0f, 0.0f, 0.0f}, bg = 0.0f, bz1 = 0.0f, bz2 = 0.0f;
+            float zp = 0.0f, q1 = 0.0f, q2 = 0.0f, bg = 0.0f, bz1 = 0.0f, bz2 = 0.0f;
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) qg[k] += q[k][d] * C::phi[d][g];
-                bg += bath[d] * C::phi[d][g];
-                bz1 += bath[d] * C::dphi0[d][g];
-                bz2 += bath[d] * C::dphi1[d][g];
+                if (d > 0) zp = fmac_tab(q[0][d], C::phi[d][g], zp);
+                q1 = fmac_tab(q[1][d], C::phi[d][g], q1);
+                q2 = fmac_tab(q[2][d], C::phi[d][g], q2);
+                bg = fmac_tab(bath[d], C::phi[d][g], bg);
+                bz1 = fmac_tab(bath[d], C::dphi0[d][g], bz1);
+                bz2 = fmac_tab(bath[d], C::dphi1[d][g], bz2);
             }
-            const float h = qg[0] + bg, rh = RKDG_RCPF(h);
-            const float u = qg[1] * rh, v = qg[2] * rh;
-            const float pe = grav * qg[0] * (0.5f * qg[0] + bg);
-            const float uvh = u * qg[2];
-            const float fx[3] = {qg[1], u * qg[1] + pe, uvh};
-            const float fy[3] = {qg[2], uvh, v * qg[2] + pe};
+            const float h = (zc + bg) + zp, rh = RKDG_RCPF(h);
+            const float u = q1 * rh, v = q2 * rh;
+            const float pe = grav * zp * (h - 0.5f * zp);  // relative to g zc (zc/2 + b)
+            const float uvh = u * q2;
+            const float fx[3] = {q1, u * q1 + pe, uvh};
+            const float fy[3] = {q2, uvh, v * q2 + pe};
             float G1[3], G2[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) G1[k] = J00 * fx[k] + J01 * fy[k], G2[k] = J10 * fx[k] + J11 * fy[k];
-            const float gze = grav * qg[0];
-            float S[3] = {0.0f, gze * (bz1 * J00 + bz2 * J10), gze * (bz1 * J01 + bz2 * J11)};
+            const float gz = grav * zp;
+            float S[3] = {0.0f, gz * (bz1 * J00 + bz2 * J10), gz * (bz1 * J01 + bz2 * J11)};
             if (CF && A.fsrc) {
                 const float x = A.vxy[e] * C::psi[0][g] + A.vxy[(size_t)ne + e] * C::psi[1][g] +
                                 A.vxy[2 * (size_t)ne + e] * C::psi[2][g];
@@ -847,58 +929,88 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
                     if (Cf < A.cf) Cf = A.cf;
                 }
                 const float tau = Cf * RKDG_SQRTF(u * u + v * v) * rh;
-                S[1] -= tau * qg[1];
-                S[2] -= tau * qg[2];
+                S[1] -= tau * q1;
+                S[2] -= tau * q2;
             }
 #pragma unroll
             for (int d = 0; d < ND; ++d)
 #pragma unroll
-                for (int k = 0; k < 3; ++k)
-                    rhs[k][d] += C::idphi0[g][d] * G1[k] + C::idphi1[g][d] * G2[k] + C::ipf[g][d] * S[k];
+                for (int k = 0; k < 3; ++k) {
+                    float r = rhs[k][d];
+                    r = fmac_tab(G1[k], C::idphi0[g][d], r);
+                    r = fmac_tab(G2[k], C::idphi1[g][d], r);
+                    if (CF || k > 0) r = fmac_tab(S[k], C::ipf[g][d], r);  // S[0] = 0 without CF
+                    rhs[k][d] = r;
+                }
         }
     }
 
-    // ---- interior faces (local face order; own tables compile-time, the neighbour's from LDS)
+    // ---- interior faces (local face order; own tables compile-time, the neighbour's from LDS).
+    // Both sides form the surface trace as zc + (sum of modes 1..) in the same order, so the mass
+    // flux and lambda see identical operands on both sides and stay exactly antisymmetric.
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        if (nbv[f] < 0 || !(wet || wet_nb[f])) continue;
-        const int fn = (meta >> (4 * f + 2)) & 3;
-        const float* pn = phie_lds + fn * ND * NE;
-        const bool is_in = e < nbv[f];
-        float F[3][NE];
-#pragma unroll
-        for (int g = 0; g < NE; ++g) {
-            const int gm = NE - 1 - g;
-            float qo[3] = {0.0f, 0.0f, 0.0f}, bo = 0.0f, qn[3] = {0.0f, 0.0f, 0.0f}, bn = 0.0f;
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                const float w = C::phie[f * ND + d][g];
-                const float wn = pn[d * NE + gm];
-#pragma unroll
-                for (int k = 0; k < 3; ++k) qo[k] += q[k][d] * w, qn[k] += qn_m[f][k][d] * wn;
-                bo += bath[d] * w, bn += bn_m[f][d] * wn;
-            }
-            float Fg[3];
-            if (is_in)
-                interface_flux_fast<WD>(grav, qo, qn, bo, bn, nx[f], ny[f], wet, wet_nb[f], true, Fg);
-            else
-                interface_flux_fast<WD>(grav, qn, qo, bn, bo, nx[f], ny[f], wet_nb[f], wet, false, Fg);
-            F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
-        }
-        const float scale = sJ[f] * rdet;
-#pragma unroll
-        for (int d = 0; d < ND; ++d) {
-            float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+        float qn_next[3][ND], bn_next[ND];
+        if (f < 2) load_nb(f + 1, qn_next, bn_next);
+        if (nbv[f] >= 0 && (wet || wet_nb[f])) {
+            const int fn = (meta >> (4 * f + 2)) & 3;
+            const float* pn = phie_lds + fn * ND * NE;
+            const bool is_in = e < nbv[f];
+            const float zcn = qn_m[0][0] * PHI0;
+            const float dzr = zcn - zr;
+            float F[3][NE];
 #pragma unroll
             for (int g = 0; g < NE; ++g) {
-                const float w = C::phiew[f * ND + d][g];
-                a0 += F[0][g] * w, a1 += F[1][g] * w, a2 += F[2][g] * w;
+                const int gm = NE - 1 - g;
+                float zo = 0.0f, o1 = 0.0f, o2 = 0.0f, bo = 0.0f, zn = 0.0f, n1 = 0.0f, n2 = 0.0f, bn = 0.0f;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const float w = C::phie[f * ND + d][g];
+                    const float wn = pn[d * NE + gm];
+                    if (d > 0) zo = fmac_tab(q[0][d], w, zo), zn = fmaf(qn_m[0][d], wn, zn);
+                    o1 = fmac_tab(q[1][d], w, o1), o2 = fmac_tab(q[2][d], w, o2), bo = fmac_tab(bath[d], w, bo);
+                    n1 = fmaf(qn_m[1][d], wn, n1), n2 = fmaf(qn_m[2][d], wn, n2), bn = fmaf(bn_m[d], wn, bn);
+                }
+                const float own[3] = {zc + zo, o1, o2}, nbs[3] = {zcn + zn, n1, n2};
+                // surface elevations relative to zr, and the pressures relative to g zr (zr/2 + bo)
+                const float zpo = wet ? zo : own[0];
+                const float zpn = dzr + zn;
+                const float B = is_in ? bo : bn;        // the LLF uses the in-side bathymetry
+                const float D = is_in ? 0.0f : zr * (bn - bo);
+                const float po = grav * (zpo * (zr + 0.5f * zpo + B) + D);
+                const float pnb = grav * (zpn * (zr + 0.5f * zpn + B) + D);
+                const float rref = grav * zr * (0.5f * zr + bo);
+                float Fg[3];
+                if (is_in)
+                    interface_flux_fast<WD>(grav, own, nbs, po, pnb, bo, bn, nx[f], ny[f], wet, wet_nb[f], true, rref, Fg);
+                else
+                    interface_flux_fast<WD>(grav, nbs, own, pnb, po, bn, bo, nx[f], ny[f], wet_nb[f], wet, false, rref, Fg);
+                F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
-            rhs[0][d] -= scale * a0, rhs[1][d] -= scale * a1, rhs[2][d] -= scale * a2;
+            const float scale = sJ[f] * rdet;
+#pragma unroll
+            for (int g = 0; g < NE; ++g) F[0][g] *= scale, F[1][g] *= scale, F[2][g] *= scale;
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float r = rhs[k][d];
+#pragma unroll
+                    for (int g = 0; g < NE; ++g) r = fmac_tab(F[k][g], -C::phiew[f * ND + d][g], r);
+                    rhs[k][d] = r;
+                }
+        }
+        if (f < 2) {
+#pragma unroll
+            for (int d = 0; d < ND; ++d) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) qn_m[k][d] = qn_next[k][d];
+                bn_m[d] = bn_next[d];
+            }
         }
     }
 
-    // ---- boundary faces (rolled loop, tables from LDS)
+    // ---- boundary faces (only for wet elements; rolled loop, tables from LDS)
     if (wet && ((nbv[0] | nbv[1] | nbv[2]) < 0)) {
 #pragma unroll 1
         for (int f = 0; f < 3; ++f) {
@@ -913,37 +1025,41 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
             float F[3][NE];
 #pragma unroll
             for (int g = 0; g < NE; ++g) {
-                float qi[3] = {0.0f, 0.0f, 0.0f}, bo = 0.0f;
+                float zo = 0.0f, o1 = 0.0f, o2 = 0.0f, bo = 0.0f;
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
                     const float w = pf[d * NE + g];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) qi[k] += q[k][d] * w;
-                    bo += bath[d] * w;
+                    if (d > 0) zo = fmaf(q[0][d], w, zo);
+                    o1 = fmaf(q[1][d], w, o1), o2 = fmaf(q[2][d], w, o2), bo = fmaf(bath[d], w, bo);
                 }
-                float qe[3];
+                const float qi[3] = {zc + zo, o1, o2};
+                float qe[3], zpe = zo;  // exterior elevation relative to zc (land, flow: the interior one)
                 if (bc == 0) {
                     const float tx = -fny, ty = fnx;
                     const float qn_ex = -(qi[1] * fnx + qi[2] * fny), qt_ex = qi[1] * tx + qi[2] * ty;
                     qe[0] = qi[0], qe[1] = qn_ex * fnx + qt_ex * tx, qe[2] = qn_ex * fny + qt_ex * ty;
                 } else {
                     exterior_state<float, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe);
+                    if (bc != 2) zpe = qe[0] - zc;  // tide, function: the exterior elevation is given
                 }
+                const float po = grav * zo * (zc + 0.5f * zo + bo);
+                const float pe = grav * zpe * (zc + 0.5f * zpe + bo);
                 float Fg[3];
-                llf_fast(grav, qi, qe, bo, fnx, fny, Fg);
+                llf_rel(grav, qi, qe, bo, po, pe, fnx, fny, Fg);
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
             const float scale = fsJ * rdet;
 #pragma unroll
-            for (int d = 0; d < ND; ++d) {
-                float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+            for (int g = 0; g < NE; ++g) F[0][g] *= scale, F[1][g] *= scale, F[2][g] *= scale;
 #pragma unroll
-                for (int g = 0; g < NE; ++g) {
-                    const float w = pw[d * NE + g];
-                    a0 += F[0][g] * w, a1 += F[1][g] * w, a2 += F[2][g] * w;
+            for (int d = 0; d < ND; ++d)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    float r = rhs[k][d];
+#pragma unroll
+                    for (int g = 0; g < NE; ++g) r = fmac_tab(F[k][g], -pw[d * NE + g], r);
+                    rhs[k][d] = r;
                 }
-                rhs[0][d] -= scale * a0, rhs[1][d] -= scale * a1, rhs[2][d] -= scale * a2;
-            }
         }
     }
 
@@ -953,12 +1069,11 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const float sol = rhs[k][d] * C::minv[0][d];
-            if (A.sol_out) A.sol_out[(size_t)(k * ND + d) * ne + e] = sol;
+            if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
             float acc = A.cur_a * q[k][d] + A.cur_c * sol;
             for (int i = 0; i < A.nterm; ++i) {
-                const size_t ix = (size_t)(k * ND + d) * ne + e;
-                float term = A.term_a[i] * A.term_q[i][ix];
-                if (A.term_sol[i]) term += A.term_c[i] * A.term_sol[i][ix];
+                float term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                if (A.term_sol[i]) term += A.term_c[i] * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc += term;
             }
             nq[k][d] = acc;

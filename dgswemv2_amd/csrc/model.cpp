@@ -121,6 +121,9 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
     M.reference_face_order = reference_face_order;
     if (md.n_elements <= 0 || md.n_nodes <= 0 || !md.node_xyz || !md.elt_nodes) return "empty mesh";
     if (prm.p < 1 || prm.p > 2) return "polynomial order outside the device-built set {1, 2}";
+    // the kernels address every element-fastest array with 32-bit byte offsets
+    if ((size_t)md.n_elements * 3 * 6 * sizeof(double) >= ((size_t)1 << 32))
+        return "mesh too large: state arrays exceed 4 GiB of 32-bit buffer offsets";
     if (!M.stepper.init(prm.nstages, prm.order, prm.dt, prm.ramp_duration)) return "invalid Runge-Kutta method";
     M.prm = prm;
     M.p = prm.p;
