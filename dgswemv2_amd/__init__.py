@@ -14,13 +14,13 @@ import numpy as np
 from . import _native as N
 from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_TIDE, FRICTION_CHEZY, FRICTION_MANNING,
                       FRICTION_NONE, PROBLEM_DAM_BREAK, PROBLEM_MANUFACTURED, PROBLEM_NONE,
-                      PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_PLAIN, RkdgError)
+                      PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
     "Mesh", "quad_mesh", "make_params", "Simulation", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
-    "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK",
+    "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
 ]
 
 

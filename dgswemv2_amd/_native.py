@@ -17,7 +17,7 @@ RKDG_OK, RKDG_ERR_ARG, RKDG_ERR_HIP, RKDG_ERR_NAN, RKDG_ERR_UNSUPPORTED, RKDG_ER
 STATUS_NAMES = {0: "RKDG_OK", 1: "RKDG_ERR_ARG", 2: "RKDG_ERR_HIP", 3: "RKDG_ERR_NAN",
                 4: "RKDG_ERR_UNSUPPORTED", 5: "RKDG_ERR_NO_DEVICE"}
 # rkdg_precision
-RKDG_FP64_PLAIN, RKDG_FP32_BULK = 0, 1
+RKDG_FP64_PLAIN, RKDG_FP32_BULK, RKDG_FP64_FAST = 0, 1, 2
 # boundary codes
 BC_LAND, BC_TIDE, BC_FLOW, BC_FUNCTION = 0, 1, 2, 3
 # problem function sets

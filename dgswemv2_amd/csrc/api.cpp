@@ -62,7 +62,19 @@ struct Engine final : EngineBase {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> stage_ev, lim_ev;
 
-    Engine(rkdg::Model& m, hipStream_t s) : M(m), stream(s) {
+    bool fast;  // fast form (FP32 bulk, FP64 fast) instead of the reference-order form
+
+    hipError_t launch_plain(const rkdg::StageArgs<Real>& A) {
+        if constexpr (sizeof(Real) == 8)
+            return rkdg::launch_stage<Real, false>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block);
+        else
+            return hipErrorInvalidValue;  // FP32 exists only in the fast form
+    }
+    hipError_t launch_fast(const rkdg::StageArgs<Real>& A) {
+        return rkdg::launch_stage<Real, true>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block);
+    }
+
+    Engine(rkdg::Model& m, hipStream_t s, bool fast_form) : M(m), stream(s), fast(fast_form) {
         if (const char* b = std::getenv("RKDG_BLOCK")) block = std::atoi(b);  // tuning experiments
         if (block < 64 || block > 256 || block % 64) block = 128;
     }
@@ -184,7 +196,7 @@ struct Engine final : EngineBase {
 
         hipEvent_t e0 = nullptr, e1 = nullptr;
         if (timing) e0 = take_event(), hipEventRecord(e0, stream);
-        hipError_t e = rkdg::launch_stage<Real>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block);
+        hipError_t e = (fast ? launch_fast(A) : launch_plain(A));
         if (e != hipSuccess) return fail("stage kernel launch", e), RKDG_ERR_HIP;
         if (timing) e1 = take_event(), hipEventRecord(e1, stream), stage_ev.push_back({e0, e1});
         ++stages_timed;
@@ -320,6 +332,10 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
         return RKDG_ERR_ARG;
     }
     *out = nullptr;
+    if (precision != RKDG_FP64_PLAIN && precision != RKDG_FP32_BULK && precision != RKDG_FP64_FAST) {
+        g_create_error = "unknown precision";
+        return RKDG_ERR_ARG;
+    }
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
         g_create_error = "no HIP device visible";
@@ -330,7 +346,7 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
     c->precision = precision;
     std::string msg;
     try {
-        msg = rkdg::build_model(*mesh, *params, c->model, precision != RKDG_FP32_BULK);
+        msg = rkdg::build_model(*mesh, *params, c->model, precision == RKDG_FP64_PLAIN);
     } catch (const std::exception& ex) {
         msg = ex.what();
     }
@@ -343,9 +359,9 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
         return RKDG_ERR_HIP;
     }
     if (precision == RKDG_FP32_BULK)
-        c->eng.reset(new Engine<float>(c->model, c->stream));
+        c->eng.reset(new Engine<float>(c->model, c->stream, true));
     else
-        c->eng.reset(new Engine<double>(c->model, c->stream));
+        c->eng.reset(new Engine<double>(c->model, c->stream, precision == RKDG_FP64_FAST));
     rkdg_status s = c->eng->init();
     if (s != RKDG_OK) {
         g_create_error = c->eng->err;

@@ -708,93 +708,130 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
 // evaluate an interface flux with identical operands (in side = lower element ID), so fluxes
 // stay exactly antisymmetric and mass is conserved to roundoff.
 
-#ifdef __HIP_DEVICE_COMPILE__
-#define RKDG_RCPF(x) __builtin_amdgcn_rcpf(x)
-#define RKDG_SQRTF(x) __builtin_amdgcn_sqrtf(x)
-#define RKDG_CBRT_RCPF(x) __builtin_amdgcn_exp2f(-0.333333343f * __builtin_amdgcn_logf(x))
-#else
-#define RKDG_RCPF(x) (1.0f / (x))
-#define RKDG_SQRTF(x) std::sqrt(x)
-#define RKDG_CBRT_RCPF(x) std::pow((x), -1.0f / 3.0f)
-#endif
+// Fast-form elementary operations.  FP32: hardware reciprocal / square root / exp2-log2
+// (about 1 ulp).  FP64: hardware reciprocal refined by two Newton steps, library square root,
+// and x^(-1/3) from an FP32 estimate refined by two Newton steps.
+RKDG_HD inline float fmaR(float a, float b, float c) { return fmaf(a, b, c); }
+RKDG_HD inline double fmaR(double a, double b, double c) { return fma(a, b, c); }
+RKDG_HD inline float absR(float a) { return fabsf(a); }
+RKDG_HD inline double absR(double a) { return fabs(a); }
 
-RKDG_HD inline void llf_fast(float grav, const float* qi, const float* qx, float bath, float nx, float ny, float* F) {
-    const float h_in = qi[0] + bath, h_ex = qx[0] + bath;
-    const float r_in = RKDG_RCPF(h_in), r_ex = RKDG_RCPF(h_ex);
-    const float u_in = qi[1] * r_in, v_in = qi[2] * r_in;
-    const float u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
-    const float un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
-    const float lam = rmax(fabsf(un_in) + RKDG_SQRTF(grav * h_in), fabsf(un_ex) + RKDG_SQRTF(grav * h_ex));
-    const float pe_in = grav * qi[0] * (0.5f * qi[0] + bath), pe_ex = grav * qx[0] * (0.5f * qx[0] + bath);
-    const float fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
-    const float fi1 = u_in * fi0 + pe_in * nx, fe1 = u_ex * fe0 + pe_ex * nx;
-    const float fi2 = v_in * fi0 + pe_in * ny, fe2 = v_ex * fe0 + pe_ex * ny;
-    F[0] = 0.5f * (fi0 + fe0 + lam * (qi[0] - qx[0]));
-    F[1] = 0.5f * (fi1 + fe1 + lam * (qi[1] - qx[1]));
-    F[2] = 0.5f * (fi2 + fe2 + lam * (qi[2] - qx[2]));
+RKDG_HD inline float rcpR(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+RKDG_HD inline double rcpR(double x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return fma(r, fma(-x, r, 1.0), r);
+#else
+    return 1.0 / x;
+#endif
+}
+RKDG_HD inline float sqrtR(float x) {
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return std::sqrt(x);
+#endif
+}
+RKDG_HD inline double sqrtR(double x) { return sqrt(x); }
+RKDG_HD inline float cbrt_rcpR(float x) {  // x^(-1/3)
+#ifdef __HIP_DEVICE_COMPILE__
+    return __builtin_amdgcn_exp2f(-0.333333343f * __builtin_amdgcn_logf(x));
+#else
+    return std::pow(x, -1.0f / 3.0f);
+#endif
+}
+RKDG_HD inline double cbrt_rcpR(double x) {  // x^(-1/3): y <- y (4 - x y^3) / 3, quadratic
+    double y = (double)cbrt_rcpR((float)x);
+    y = y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
+    return y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
+}
+
+template <typename Real>
+RKDG_HD inline void llf_fast(Real grav, const Real* qi, const Real* qx, Real bath, Real nx, Real ny, Real* F) {
+    const Real h_in = qi[0] + bath, h_ex = qx[0] + bath;
+    const Real r_in = rcpR(h_in), r_ex = rcpR(h_ex);
+    const Real u_in = qi[1] * r_in, v_in = qi[2] * r_in;
+    const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
+    const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
+    const Real lam = rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
+    const Real pe_in = grav * qi[0] * (Real(0.5) * qi[0] + bath), pe_ex = grav * qx[0] * (Real(0.5) * qx[0] + bath);
+    const Real fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
+    const Real fi1 = u_in * fi0 + pe_in * nx, fe1 = u_ex * fe0 + pe_ex * nx;
+    const Real fi2 = v_in * fi0 + pe_in * ny, fe2 = v_ex * fe0 + pe_ex * ny;
+    F[0] = Real(0.5) * (fi0 + fe0 + lam * (qi[0] - qx[0]));
+    F[1] = Real(0.5) * (fi1 + fe1 + lam * (qi[1] - qx[1]));
+    F[2] = Real(0.5) * (fi2 + fe2 + lam * (qi[2] - qx[2]));
 }
 
 // The same LLF flux with the pressures of both states given relative to a hydrostatic
 // reference r: pi = pe_in - r, px = pe_ex - r.  The mass component and lambda are the plain
 // ones; the momentum components come out as F - r n.
-RKDG_HD inline void llf_rel(float grav, const float* qi, const float* qx, float bath, float pi, float px, float nx,
-                            float ny, float* F) {
-    const float h_in = qi[0] + bath, h_ex = qx[0] + bath;
-    const float r_in = RKDG_RCPF(h_in), r_ex = RKDG_RCPF(h_ex);
-    const float u_in = qi[1] * r_in, v_in = qi[2] * r_in;
-    const float u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
-    const float un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
-    const float lam = rmax(fabsf(un_in) + RKDG_SQRTF(grav * h_in), fabsf(un_ex) + RKDG_SQRTF(grav * h_ex));
-    const float fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
-    F[0] = 0.5f * (fi0 + fe0 + lam * (qi[0] - qx[0]));
-    F[1] = 0.5f * (u_in * fi0 + u_ex * fe0 + (pi + px) * nx + lam * (qi[1] - qx[1]));
-    F[2] = 0.5f * (v_in * fi0 + v_ex * fe0 + (pi + px) * ny + lam * (qi[2] - qx[2]));
+template <typename Real>
+RKDG_HD inline void llf_rel(Real grav, const Real* qi, const Real* qx, Real bath, Real pi, Real px, Real nx,
+                            Real ny, Real* F) {
+    const Real h_in = qi[0] + bath, h_ex = qx[0] + bath;
+    const Real r_in = rcpR(h_in), r_ex = rcpR(h_ex);
+    const Real u_in = qi[1] * r_in, v_in = qi[2] * r_in;
+    const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
+    const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
+    const Real lam = rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
+    const Real fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
+    F[0] = Real(0.5) * (fi0 + fe0 + lam * (qi[0] - qx[0]));
+    F[1] = Real(0.5) * (u_in * fi0 + u_ex * fe0 + (pi + px) * nx + lam * (qi[1] - qx[1]));
+    F[2] = Real(0.5) * (v_in * fi0 + v_ex * fe0 + (pi + px) * ny + lam * (qi[2] - qx[2]));
 }
 
 // Interface flux of one side in the relative-pressure form: pi/px as for llf_rel, r the
 // caller's hydrostatic reference at this point.  Returns the caller's flux (w.r.t. its outward
 // normal) with momentum relative to r.  The wet/dry fixes of rkdg_swe_is_internal.hpp:47-90
 // (rare) are applied to the plain flux, rebuilt as F_rel + r n.
-template <bool WD>
-RKDG_HD inline void interface_flux_fast(float grav, const float* qi, const float* qx, float pi, float px, float bi,
-                                        float bx, float nx, float ny, bool wet_in, bool wet_ex, bool want_in, float r,
-                                        float* F) {
-    float Fi[3];
+template <typename Real, bool WD>
+RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* qx, Real pi, Real px, Real bi,
+                                        Real bx, Real nx, Real ny, bool wet_in, bool wet_ex, bool want_in, Real r,
+                                        Real* F) {
+    Real Fi[3];
     llf_rel(grav, qi, qx, bi, pi, px, nx, ny, Fi);
-    if (WD && ((Fi[0] > 1e-12f && !(wet_in && wet_ex)) || (Fi[0] < -1e-12f && !(wet_in && wet_ex)))) {
+    if (WD && ((Fi[0] > Real(1e-12) && !(wet_in && wet_ex)) || (Fi[0] < -Real(1e-12) && !(wet_in && wet_ex)))) {
         Fi[1] += r * nx, Fi[2] += r * ny;
-        float Fe[3] = {-Fi[0], -Fi[1], -Fi[2]};
-        if (Fi[0] > 1e-12f) {
+        Real Fe[3] = {-Fi[0], -Fi[1], -Fi[2]};
+        if (Fi[0] > Real(1e-12)) {
             if (!wet_in) {
                 Fi[0] = 0, Fi[1] = 0, Fi[2] = 0;
-                const float tx = ny, ty = -nx, mx = -nx, my = -ny;
-                const float qn_ex = -(qx[1] * mx + qx[2] * my), qt_ex = qx[1] * tx + qx[2] * ty;
-                const float qe[3] = {qx[0], qn_ex * mx + qt_ex * tx, qn_ex * my + qt_ex * ty};
+                const Real tx = ny, ty = -nx, mx = -nx, my = -ny;
+                const Real qn_ex = -(qx[1] * mx + qx[2] * my), qt_ex = qx[1] * tx + qx[2] * ty;
+                const Real qe[3] = {qx[0], qn_ex * mx + qt_ex * tx, qn_ex * my + qt_ex * ty};
                 llf_fast(grav, qx, qe, bx, mx, my, Fe);
             } else {  // !wet_ex
-                const float f0 = Fi[0];
-                llf_fast(0.0f, qx, qi, bx, -nx, -ny, Fe);
+                const Real f0 = Fi[0];
+                llf_fast(Real(0.0), qx, qi, bx, -nx, -ny, Fe);
                 Fe[0] = -f0;
             }
         } else {
             if (!wet_ex) {
                 Fe[0] = 0, Fe[1] = 0, Fe[2] = 0;
-                const float tx = -ny, ty = nx;
-                const float qn_ex = -(qi[1] * nx + qi[2] * ny), qt_ex = qi[1] * tx + qi[2] * ty;
-                const float qe[3] = {qi[0], qn_ex * nx + qt_ex * tx, qn_ex * ny + qt_ex * ty};
+                const Real tx = -ny, ty = nx;
+                const Real qn_ex = -(qi[1] * nx + qi[2] * ny), qt_ex = qi[1] * tx + qi[2] * ty;
+                const Real qe[3] = {qi[0], qn_ex * nx + qt_ex * tx, qn_ex * ny + qt_ex * ty};
                 llf_fast(grav, qi, qe, bi, nx, ny, Fi);
             } else {  // !wet_in
-                const float fe0 = Fe[0];
-                llf_fast(0.0f, qi, qx, bi, nx, ny, Fi);
+                const Real fe0 = Fe[0];
+                llf_fast(Real(0.0), qi, qx, bi, nx, ny, Fi);
                 Fi[0] = -fe0;
             }
         }
-        const float s = want_in ? 1.0f : -1.0f;
-        const float* src = want_in ? Fi : Fe;
+        const Real s = want_in ? Real(1.0) : -Real(1.0);
+        const Real* src = want_in ? Fi : Fe;
         F[0] = src[0], F[1] = src[1] - r * s * nx, F[2] = src[2] - r * s * ny;
         return;
     }
-    const float s = want_in ? 1.0f : -1.0f;
+    const Real s = want_in ? Real(1.0) : -Real(1.0);
     F[0] = s * Fi[0], F[1] = s * Fi[1], F[2] = s * Fi[2];
 }
 
@@ -825,33 +862,37 @@ RKDG_HD inline void exterior_state(const StageArgs<Real>& A, int bc, int e, int 
 
 // r + a * c for a compile-time table entry c; entries that are exactly zero (derivatives of the
 // constant mode, basis functions vanishing at a point) fold away at compile time.
-RKDG_HD inline float fmac_tab(float a, float c, float r) { return c != 0.0f ? fmaf(a, c, r) : r; }
+template <typename Real>
+RKDG_HD inline Real fmac_tab(Real a, Real c, Real r) { return c != Real(0) ? fmaR(a, c, r) : r; }
 
-template <int P, bool WD, bool CF>
-RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* phie_lds, const float* phiew_lds,
+template <typename Real, int P, bool WD, bool CF>
+RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phie_lds, const Real* phiew_lds,
                                        int e) {
-    using C = ConstTables<float, P>;
+    using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
-    const float grav = A.g;
+    const Real grav = A.g;
 
     // ---- every load this element needs, issued before any arithmetic
-    float q[3][ND], bath[ND];
+    Real q[3][ND], bath[ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int d = 0; d < ND; ++d) q[k][d] = ldc(A.q_s, ne, k * ND + d, e);
 #pragma unroll
     for (int d = 0; d < ND; ++d) bath[d] = ldc(A.bath, ne, d, e);
-    const float J00 = ldc(A.geom, ne, 0, e), J01 = ldc(A.geom, ne, 1, e), J10 = ldc(A.geom, ne, 2, e),
+    const Real J00 = ldc(A.geom, ne, 0, e), J01 = ldc(A.geom, ne, 1, e), J10 = ldc(A.geom, ne, 2, e),
                 J11 = ldc(A.geom, ne, 3, e), adet = ldc(A.geom, ne, 4, e);
     const uint32_t meta = ldc(A.meta, ne, 0, e);
     int nbv[3], ed[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) nbv[f] = ldc(A.nb, ne, f, e), ed[f] = ldc(A.edge, ne, f, e);
-    // Neighbour modes are loaded one face ahead: face 0's before the volume integral, face f+1's
-    // while face f computes, so two faces' worth of registers are live at most.
-    auto load_nb = [&](int f, float (&qm)[3][ND], float (&bm)[ND]) {
+    // FP32 loads neighbour modes one face ahead (face 0's before the volume integral, face f+1's
+    // while face f computes; two faces' worth of registers live at most) and the edge geometry up
+    // front.  FP64 has no registers to spare: it loads both at the face itself (occupancy 2
+    // instead of 1).
+    constexpr bool kAhead = sizeof(Real) == 4;
+    auto load_nb = [&](int f, Real (&qm)[3][ND], Real (&bm)[ND]) {
         const int n = nbv[f] >= 0 ? nbv[f] : e;  // boundary faces read their own (cached) modes
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
@@ -860,18 +901,20 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
             bm[d] = ldc(A.bath, ne, d, n);
         }
     };
-    float qn_m[3][ND], bn_m[ND];
-    load_nb(0, qn_m, bn_m);
-    float nx[3], ny[3], sJ[3];
+    Real qn_m[3][ND], bn_m[ND];
+    if (kAhead) load_nb(0, qn_m, bn_m);
+    Real nx[3], ny[3], sJ[3];
+    if (kAhead)
 #pragma unroll
-    for (int f = 0; f < 3; ++f)
-        nx[f] = ldc(A.egeom, A.nedge, 0, ed[f]), ny[f] = ldc(A.egeom, A.nedge, 1, ed[f]), sJ[f] = ldc(A.egeom, A.nedge, 2, ed[f]);
+        for (int f = 0; f < 3; ++f)
+            nx[f] = ldc(A.egeom, A.nedge, 0, ed[f]), ny[f] = ldc(A.egeom, A.nedge, 1, ed[f]),
+            sJ[f] = ldc(A.egeom, A.nedge, 2, ed[f]);
     const bool wet = WD ? (A.wet_in[e] != 0) : true;
     bool wet_nb[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) wet_nb[f] = (WD && nbv[f] >= 0) ? (A.wet_in[nbv[f]] != 0) : true;
-    const float gmn2 = A.friction == 2 ? ldc(A.gmn2, ne, 0, e) : 0.0f;
-    const float rdet = RKDG_RCPF(adet);
+    const Real gmn2 = A.friction == 2 ? ldc(A.gmn2, ne, 0, e) : Real(0.0);
+    const Real rdet = rcpR(adet);
 
     // ---- hydrostatic reference.  A wet element works with its momentum pressure relative to
     // g zr (zr/2 + b), zr = its mean surface elevation.  With the quadratures exact for these
@@ -879,20 +922,20 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
     // the volume uses pe - g zr (zr/2 + b) = g z' (h - z'/2) with z' = ze - zr, the source
     // g z' grad b, and every face subtracts g zr (zr/2 + b_own) n.  Same discretisation; the
     // large canceling terms (g ze b with b up to kilometres) never form in FP32.
-    constexpr float PHI0 = C::phi[0][0];  // the constant mode's value
-    const float zc = q[0][0] * PHI0;      // mean surface elevation (mode 0)
-    const float zr = wet ? zc : 0.0f;     // dry elements keep the plain form (no volume term)
+    constexpr Real PHI0 = C::phi[0][0];  // the constant mode's value
+    const Real zc = q[0][0] * PHI0;      // mean surface elevation (mode 0)
+    const Real zr = wet ? zc : Real(0.0);     // dry elements keep the plain form (no volume term)
 
     // ---- volume and source: rhs (unscaled by |det J|) = sum_g [idphi . (J^-T F) + ipf . S]
-    float rhs[3][ND];
+    Real rhs[3][ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
-        for (int d = 0; d < ND; ++d) rhs[k][d] = 0.0f;
+        for (int d = 0; d < ND; ++d) rhs[k][d] = Real(0.0);
     if (wet) {
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            float zp = 0.0f, q1 = 0.0f, q2 = 0.0f, bg = 0.0f, bz1 = 0.0f, bz2 = 0.0f;
+            Real zp = Real(0.0), q1 = Real(0.0), q2 = Real(0.0), bg = Real(0.0), bz1 = Real(0.0), bz2 = Real(0.0);
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 if (d > 0) zp = fmac_tab(q[0][d], C::phi[d][g], zp);
@@ -902,33 +945,33 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
                 bz1 = fmac_tab(bath[d], C::dphi0[d][g], bz1);
                 bz2 = fmac_tab(bath[d], C::dphi1[d][g], bz2);
             }
-            const float h = (zc + bg) + zp, rh = RKDG_RCPF(h);
-            const float u = q1 * rh, v = q2 * rh;
-            const float pe = grav * zp * (h - 0.5f * zp);  // relative to g zc (zc/2 + b)
-            const float uvh = u * q2;
-            const float fx[3] = {q1, u * q1 + pe, uvh};
-            const float fy[3] = {q2, uvh, v * q2 + pe};
-            float G1[3], G2[3];
+            const Real h = (zc + bg) + zp, rh = rcpR(h);
+            const Real u = q1 * rh, v = q2 * rh;
+            const Real pe = grav * zp * (h - Real(0.5) * zp);  // relative to g zc (zc/2 + b)
+            const Real uvh = u * q2;
+            const Real fx[3] = {q1, u * q1 + pe, uvh};
+            const Real fy[3] = {q2, uvh, v * q2 + pe};
+            Real G1[3], G2[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) G1[k] = J00 * fx[k] + J01 * fy[k], G2[k] = J10 * fx[k] + J11 * fy[k];
-            const float gz = grav * zp;
-            float S[3] = {0.0f, gz * (bz1 * J00 + bz2 * J10), gz * (bz1 * J01 + bz2 * J11)};
+            const Real gz = grav * zp;
+            Real S[3] = {Real(0.0), gz * (bz1 * J00 + bz2 * J10), gz * (bz1 * J01 + bz2 * J11)};
             if (CF && A.fsrc) {
-                const float x = A.vxy[e] * C::psi[0][g] + A.vxy[(size_t)ne + e] * C::psi[1][g] +
+                const Real x = A.vxy[e] * C::psi[0][g] + A.vxy[(size_t)ne + e] * C::psi[1][g] +
                                 A.vxy[2 * (size_t)ne + e] * C::psi[2][g];
-                const float y = A.vxy[3 * (size_t)ne + e] * C::psi[0][g] + A.vxy[4 * (size_t)ne + e] * C::psi[1][g] +
+                const Real y = A.vxy[3 * (size_t)ne + e] * C::psi[0][g] + A.vxy[4 * (size_t)ne + e] * C::psi[1][g] +
                                 A.vxy[5 * (size_t)ne + e] * C::psi[2][g];
-                float fsv[3];
-                problem_source<float>(A.t, x, y, fsv);
+                Real fsv[3];
+                problem_source<Real>(A.t, x, y, fsv);
                 S[0] += fsv[0], S[1] += fsv[1], S[2] += fsv[2];
             }
             if (A.friction) {
-                float Cf = A.cf;
+                Real Cf = A.cf;
                 if (A.friction == 2) {
-                    Cf = gmn2 * RKDG_CBRT_RCPF(h);
+                    Cf = gmn2 * cbrt_rcpR(h);
                     if (Cf < A.cf) Cf = A.cf;
                 }
-                const float tau = Cf * RKDG_SQRTF(u * u + v * v) * rh;
+                const Real tau = Cf * sqrtR(u * u + v * v) * rh;
                 S[1] -= tau * q1;
                 S[2] -= tau * q2;
             }
@@ -936,7 +979,7 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
             for (int d = 0; d < ND; ++d)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    float r = rhs[k][d];
+                    Real r = rhs[k][d];
                     r = fmac_tab(G1[k], C::idphi0[g][d], r);
                     r = fmac_tab(G2[k], C::idphi1[g][d], r);
                     if (CF || k > 0) r = fmac_tab(S[k], C::ipf[g][d], r);  // S[0] = 0 without CF
@@ -950,57 +993,61 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
     // flux and lambda see identical operands on both sides and stay exactly antisymmetric.
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        float qn_next[3][ND], bn_next[ND];
-        if (f < 2) load_nb(f + 1, qn_next, bn_next);
+        Real qn_next[3][ND], bn_next[ND];
+        if (kAhead && f < 2) load_nb(f + 1, qn_next, bn_next);
+        if (!kAhead) load_nb(f, qn_m, bn_m);
         if (nbv[f] >= 0 && (wet || wet_nb[f])) {
+            const Real fnx = kAhead ? nx[f] : ldc(A.egeom, A.nedge, 0, ed[f]);
+            const Real fny = kAhead ? ny[f] : ldc(A.egeom, A.nedge, 1, ed[f]);
+            const Real fsJ = kAhead ? sJ[f] : ldc(A.egeom, A.nedge, 2, ed[f]);
             const int fn = (meta >> (4 * f + 2)) & 3;
-            const float* pn = phie_lds + fn * ND * NE;
+            const Real* pn = phie_lds + fn * ND * NE;
             const bool is_in = e < nbv[f];
-            const float zcn = qn_m[0][0] * PHI0;
-            const float dzr = zcn - zr;
-            float F[3][NE];
+            const Real zcn = qn_m[0][0] * PHI0;
+            const Real dzr = zcn - zr;
+            Real F[3][NE];
 #pragma unroll
             for (int g = 0; g < NE; ++g) {
                 const int gm = NE - 1 - g;
-                float zo = 0.0f, o1 = 0.0f, o2 = 0.0f, bo = 0.0f, zn = 0.0f, n1 = 0.0f, n2 = 0.0f, bn = 0.0f;
+                Real zo = Real(0.0), o1 = Real(0.0), o2 = Real(0.0), bo = Real(0.0), zn = Real(0.0), n1 = Real(0.0), n2 = Real(0.0), bn = Real(0.0);
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const float w = C::phie[f * ND + d][g];
-                    const float wn = pn[d * NE + gm];
-                    if (d > 0) zo = fmac_tab(q[0][d], w, zo), zn = fmaf(qn_m[0][d], wn, zn);
+                    const Real w = C::phie[f * ND + d][g];
+                    const Real wn = pn[d * NE + gm];
+                    if (d > 0) zo = fmac_tab(q[0][d], w, zo), zn = fmaR(qn_m[0][d], wn, zn);
                     o1 = fmac_tab(q[1][d], w, o1), o2 = fmac_tab(q[2][d], w, o2), bo = fmac_tab(bath[d], w, bo);
-                    n1 = fmaf(qn_m[1][d], wn, n1), n2 = fmaf(qn_m[2][d], wn, n2), bn = fmaf(bn_m[d], wn, bn);
+                    n1 = fmaR(qn_m[1][d], wn, n1), n2 = fmaR(qn_m[2][d], wn, n2), bn = fmaR(bn_m[d], wn, bn);
                 }
-                const float own[3] = {zc + zo, o1, o2}, nbs[3] = {zcn + zn, n1, n2};
+                const Real own[3] = {zc + zo, o1, o2}, nbs[3] = {zcn + zn, n1, n2};
                 // surface elevations relative to zr, and the pressures relative to g zr (zr/2 + bo)
-                const float zpo = wet ? zo : own[0];
-                const float zpn = dzr + zn;
-                const float B = is_in ? bo : bn;        // the LLF uses the in-side bathymetry
-                const float D = is_in ? 0.0f : zr * (bn - bo);
-                const float po = grav * (zpo * (zr + 0.5f * zpo + B) + D);
-                const float pnb = grav * (zpn * (zr + 0.5f * zpn + B) + D);
-                const float rref = grav * zr * (0.5f * zr + bo);
-                float Fg[3];
+                const Real zpo = wet ? zo : own[0];
+                const Real zpn = dzr + zn;
+                const Real B = is_in ? bo : bn;        // the LLF uses the in-side bathymetry
+                const Real D = is_in ? Real(0.0) : zr * (bn - bo);
+                const Real po = grav * (zpo * (zr + Real(0.5) * zpo + B) + D);
+                const Real pnb = grav * (zpn * (zr + Real(0.5) * zpn + B) + D);
+                const Real rref = grav * zr * (Real(0.5) * zr + bo);
+                Real Fg[3];
                 if (is_in)
-                    interface_flux_fast<WD>(grav, own, nbs, po, pnb, bo, bn, nx[f], ny[f], wet, wet_nb[f], true, rref, Fg);
+                    interface_flux_fast<Real, WD>(grav, own, nbs, po, pnb, bo, bn, fnx, fny, wet, wet_nb[f], true, rref, Fg);
                 else
-                    interface_flux_fast<WD>(grav, nbs, own, pnb, po, bn, bo, nx[f], ny[f], wet_nb[f], wet, false, rref, Fg);
+                    interface_flux_fast<Real, WD>(grav, nbs, own, pnb, po, bn, bo, fnx, fny, wet_nb[f], wet, false, rref, Fg);
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
-            const float scale = sJ[f] * rdet;
+            const Real scale = fsJ * rdet;
 #pragma unroll
             for (int g = 0; g < NE; ++g) F[0][g] *= scale, F[1][g] *= scale, F[2][g] *= scale;
 #pragma unroll
             for (int d = 0; d < ND; ++d)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    float r = rhs[k][d];
+                    Real r = rhs[k][d];
 #pragma unroll
                     for (int g = 0; g < NE; ++g) r = fmac_tab(F[k][g], -C::phiew[f * ND + d][g], r);
                     rhs[k][d] = r;
                 }
         }
-        if (f < 2) {
+        if (kAhead && f < 2) {
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
 #pragma unroll
@@ -1016,46 +1063,46 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
         for (int f = 0; f < 3; ++f) {
             if (nbv[f] >= 0) continue;
             const int bc = -1 - nbv[f];
-            const float* pf = phie_lds + f * ND * NE;
-            const float* pw = phiew_lds + f * ND * NE;
-            const float fnx = f == 0 ? nx[0] : (f == 1 ? nx[1] : nx[2]);
-            const float fny = f == 0 ? ny[0] : (f == 1 ? ny[1] : ny[2]);
-            const float fsJ = f == 0 ? sJ[0] : (f == 1 ? sJ[1] : sJ[2]);
+            const Real* pf = phie_lds + f * ND * NE;
+            const Real* pw = phiew_lds + f * ND * NE;
             const int fed = f == 0 ? ed[0] : (f == 1 ? ed[1] : ed[2]);
-            float F[3][NE];
+            const Real fnx = kAhead ? (f == 0 ? nx[0] : (f == 1 ? nx[1] : nx[2])) : ldc(A.egeom, A.nedge, 0, fed);
+            const Real fny = kAhead ? (f == 0 ? ny[0] : (f == 1 ? ny[1] : ny[2])) : ldc(A.egeom, A.nedge, 1, fed);
+            const Real fsJ = kAhead ? (f == 0 ? sJ[0] : (f == 1 ? sJ[1] : sJ[2])) : ldc(A.egeom, A.nedge, 2, fed);
+            Real F[3][NE];
 #pragma unroll
             for (int g = 0; g < NE; ++g) {
-                float zo = 0.0f, o1 = 0.0f, o2 = 0.0f, bo = 0.0f;
+                Real zo = Real(0.0), o1 = Real(0.0), o2 = Real(0.0), bo = Real(0.0);
 #pragma unroll
                 for (int d = 0; d < ND; ++d) {
-                    const float w = pf[d * NE + g];
-                    if (d > 0) zo = fmaf(q[0][d], w, zo);
-                    o1 = fmaf(q[1][d], w, o1), o2 = fmaf(q[2][d], w, o2), bo = fmaf(bath[d], w, bo);
+                    const Real w = pf[d * NE + g];
+                    if (d > 0) zo = fmaR(q[0][d], w, zo);
+                    o1 = fmaR(q[1][d], w, o1), o2 = fmaR(q[2][d], w, o2), bo = fmaR(bath[d], w, bo);
                 }
-                const float qi[3] = {zc + zo, o1, o2};
-                float qe[3], zpe = zo;  // exterior elevation relative to zc (land, flow: the interior one)
+                const Real qi[3] = {zc + zo, o1, o2};
+                Real qe[3], zpe = zo;  // exterior elevation relative to zc (land, flow: the interior one)
                 if (bc == 0) {
-                    const float tx = -fny, ty = fnx;
-                    const float qn_ex = -(qi[1] * fnx + qi[2] * fny), qt_ex = qi[1] * tx + qi[2] * ty;
+                    const Real tx = -fny, ty = fnx;
+                    const Real qn_ex = -(qi[1] * fnx + qi[2] * fny), qt_ex = qi[1] * tx + qi[2] * ty;
                     qe[0] = qi[0], qe[1] = qn_ex * fnx + qt_ex * tx, qe[2] = qn_ex * fny + qt_ex * ty;
                 } else {
-                    exterior_state<float, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe);
+                    exterior_state<Real, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe);
                     if (bc != 2) zpe = qe[0] - zc;  // tide, function: the exterior elevation is given
                 }
-                const float po = grav * zo * (zc + 0.5f * zo + bo);
-                const float pe = grav * zpe * (zc + 0.5f * zpe + bo);
-                float Fg[3];
+                const Real po = grav * zo * (zc + Real(0.5) * zo + bo);
+                const Real pe = grav * zpe * (zc + Real(0.5) * zpe + bo);
+                Real Fg[3];
                 llf_rel(grav, qi, qe, bo, po, pe, fnx, fny, Fg);
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
-            const float scale = fsJ * rdet;
+            const Real scale = fsJ * rdet;
 #pragma unroll
             for (int g = 0; g < NE; ++g) F[0][g] *= scale, F[1][g] *= scale, F[2][g] *= scale;
 #pragma unroll
             for (int d = 0; d < ND; ++d)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    float r = rhs[k][d];
+                    Real r = rhs[k][d];
 #pragma unroll
                     for (int g = 0; g < NE; ++g) r = fmac_tab(F[k][g], -pw[d * NE + g], r);
                     rhs[k][d] = r;
@@ -1063,22 +1110,22 @@ RKDG_HD inline void stage_element_fast(const StageArgs<float>& A, const float* p
         }
     }
 
-    float nq[3][ND];
+    Real nq[3][ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
-            const float sol = rhs[k][d] * C::minv[0][d];
+            const Real sol = rhs[k][d] * C::minv[0][d];
             if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
-            float acc = A.cur_a * q[k][d] + A.cur_c * sol;
+            Real acc = A.cur_a * q[k][d] + A.cur_c * sol;
             for (int i = 0; i < A.nterm; ++i) {
-                float term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
                 if (A.term_sol[i]) term += A.term_c[i] * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc += term;
             }
             nq[k][d] = acc;
         }
-    finish_element<float, P, WD>(A, e, wet, nq);
+    finish_element<Real, P, WD>(A, e, wet, nq);
 }
 
 }  // namespace rkdg

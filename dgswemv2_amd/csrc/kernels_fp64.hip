@@ -3,6 +3,6 @@
 #include "kernels.inc"
 
 namespace rkdg {
-template hipError_t launch_stage<double>(int, bool, bool, const StageArgs<double>&, hipStream_t, int);
+template hipError_t launch_stage<double, false>(int, bool, bool, const StageArgs<double>&, hipStream_t, int);
 template hipError_t launch_limiter<double>(int, const LimiterArgs<double>&, hipStream_t, int);
 }  // namespace rkdg

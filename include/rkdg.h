@@ -37,8 +37,10 @@ typedef enum {
 } rkdg_status;
 
 typedef enum {
-    RKDG_FP64_PLAIN = 0, /* FP64 everywhere: parity form */
-    RKDG_FP32_BULK = 1   /* FP32 state/flux/accumulation; FP64 geometry precompute and wet/dry */
+    RKDG_FP64_PLAIN = 0, /* FP64 in the reference's operation order: bit-exact parity form */
+    RKDG_FP32_BULK = 1,  /* FP32 state/flux/accumulation; FP64 geometry precompute and wet/dry */
+    RKDG_FP64_FAST = 2   /* FP64 with fused multiply-adds, reordered sums and Newton reciprocals:
+                            within 1e-12 relative per-step L2 of the plain form */
 } rkdg_precision;
 
 /* Boundary condition codes (reference: SWE::BoundaryTypes, problem/SWE/swe_definitions.hpp:61-69). */
