@@ -12,13 +12,13 @@ import math
 import numpy as np
 
 from . import _native as N
-from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_TIDE, FRICTION_CHEZY, FRICTION_MANNING,
+from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION_CHEZY, FRICTION_MANNING,
                       FRICTION_NONE, PROBLEM_DAM_BREAK, PROBLEM_MANUFACTURED, PROBLEM_NONE,
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "make_params", "Simulation", "RkdgError", "ndof_of",
-    "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION",
+    "Mesh", "quad_mesh", "read_adcirc", "make_params", "Simulation", "RkdgError", "ndof_of",
+    "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
 ]
@@ -32,7 +32,8 @@ class Mesh:
     """ADCIRC-style mesh: nodes (x, y, depth), triangles, typed boundary edges, forcing."""
 
     def __init__(self, node_xyz, elt_nodes, bound_edges, bound_types, frequency=(), forcing_fact=(),
-                 equilib_arg=(), node_amplitude=None, node_phase=None, node_manning_n=None, _owner=None):
+                 equilib_arg=(), node_amplitude=None, node_phase=None, node_manning_n=None, levee_nodes=None,
+                 levee_param=None, _owner=None):
         self.node_xyz = np.ascontiguousarray(node_xyz, dtype=np.float64).reshape(-1, 3)
         self.elt_nodes = np.ascontiguousarray(elt_nodes, dtype=np.int32).reshape(-1, 3)
         self.bound_edges = np.ascontiguousarray(bound_edges, dtype=np.int32).reshape(-1, 2)
@@ -46,6 +47,12 @@ class Mesh:
         self.node_manning_n = None if node_manning_n is None else np.ascontiguousarray(node_manning_n, np.float64)
         if nc and (self.node_amplitude is None or self.node_phase is None):
             raise ValueError("constituents given without nodal amplitude/phase")
+        self.levee_nodes = np.zeros((0, 2), np.int32) if levee_nodes is None else \
+            np.ascontiguousarray(levee_nodes, dtype=np.int32).reshape(-1, 2)
+        self.levee_param = np.zeros((0, 3), np.float64) if levee_param is None else \
+            np.ascontiguousarray(levee_param, dtype=np.float64).reshape(-1, 3)
+        if len(self.levee_nodes) != len(self.levee_param):
+            raise ValueError("levee_nodes and levee_param differ in length")
         self._owner = _owner
         self.desc = N.MeshDesc()
         d = self.desc
@@ -60,6 +67,9 @@ class Mesh:
         d.node_amplitude = N.dptr(self.node_amplitude)
         d.node_phase = N.dptr(self.node_phase)
         d.node_manning_n = N.dptr(self.node_manning_n)
+        d.n_levee_pairs = len(self.levee_nodes)
+        d.levee_nodes = N.iptr(self.levee_nodes) if len(self.levee_nodes) else None
+        d.levee_param = N.dptr(self.levee_param) if len(self.levee_nodes) else None
 
     @property
     def n_elements(self):
@@ -99,25 +109,50 @@ def quad_mesh(x1, y1, x2, y2, m, n, bc=(BC_LAND,) * 4, pattern=0, bathy="constan
     if st != N.RKDG_OK:
         raise RkdgError(st, "mesh generation failed")
     try:
-        nn, ne = desc.n_nodes, desc.n_elements
-        xyz = np.ctypeslib.as_array(desc.node_xyz, shape=(nn * 3,)).copy()
-        elt = np.ctypeslib.as_array(desc.elt_nodes, shape=(ne * 3,)).copy()
-        nb = desc.n_bound_edges
-        be = np.ctypeslib.as_array(desc.bound_edge_nodes, shape=(nb * 2,)).copy()
-        bt = np.ctypeslib.as_array(desc.bound_edge_type, shape=(nb,)).copy()
-        nc = desc.n_constituents
-        kw = {}
-        if nc:
-            kw = dict(frequency=np.ctypeslib.as_array(desc.frequency, shape=(nc,)).copy(),
-                      forcing_fact=np.ctypeslib.as_array(desc.forcing_fact, shape=(nc,)).copy(),
-                      equilib_arg=np.ctypeslib.as_array(desc.equilib_arg, shape=(nc,)).copy(),
-                      node_amplitude=np.ctypeslib.as_array(desc.node_amplitude, shape=(nn * nc,)).copy(),
-                      node_phase=np.ctypeslib.as_array(desc.node_phase, shape=(nn * nc,)).copy())
-        if desc.node_manning_n:
-            kw["node_manning_n"] = np.ctypeslib.as_array(desc.node_manning_n, shape=(nn,)).copy()
+        return _mesh_from_desc(desc)
     finally:
         lib.rkdg_mesh_free(handle)
+
+
+def _mesh_from_desc(desc):
+    """Copies a library-owned mesh description into a Mesh."""
+    nn, ne = desc.n_nodes, desc.n_elements
+    xyz = np.ctypeslib.as_array(desc.node_xyz, shape=(nn * 3,)).copy()
+    elt = np.ctypeslib.as_array(desc.elt_nodes, shape=(ne * 3,)).copy()
+    nb = desc.n_bound_edges
+    be = np.ctypeslib.as_array(desc.bound_edge_nodes, shape=(nb * 2,)).copy() if nb else np.zeros(0, np.int32)
+    bt = np.ctypeslib.as_array(desc.bound_edge_type, shape=(nb,)).copy() if nb else np.zeros(0, np.uint8)
+    nc = desc.n_constituents
+    kw = {}
+    if nc:
+        kw = dict(frequency=np.ctypeslib.as_array(desc.frequency, shape=(nc,)).copy(),
+                  forcing_fact=np.ctypeslib.as_array(desc.forcing_fact, shape=(nc,)).copy(),
+                  equilib_arg=np.ctypeslib.as_array(desc.equilib_arg, shape=(nc,)).copy(),
+                  node_amplitude=np.ctypeslib.as_array(desc.node_amplitude, shape=(nn * nc,)).copy(),
+                  node_phase=np.ctypeslib.as_array(desc.node_phase, shape=(nn * nc,)).copy())
+    if desc.node_manning_n:
+        kw["node_manning_n"] = np.ctypeslib.as_array(desc.node_manning_n, shape=(nn,)).copy()
+    nl = desc.n_levee_pairs
+    if nl:
+        kw["levee_nodes"] = np.ctypeslib.as_array(desc.levee_nodes, shape=(nl * 2,)).copy()
+        kw["levee_param"] = np.ctypeslib.as_array(desc.levee_param, shape=(nl * 3,)).copy()
     return Mesh(xyz, elt, be, bt, **kw)
+
+
+def read_adcirc(fort14, bcis=None):
+    """Reads an ADCIRC .14 mesh and its .bcis boundary data through the library's reader
+    (AdcircFormat + SWE::Inputs::read_bcis of the reference)."""
+    lib = N.load_library()
+    handle = C.c_void_p()
+    desc = N.MeshDesc()
+    st = lib.rkdg_mesh_read_adcirc(str(fort14).encode(), None if bcis is None else str(bcis).encode(),
+                                   C.byref(handle), C.byref(desc))
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
+    try:
+        return _mesh_from_desc(desc)
+    finally:
+        lib.rkdg_mesh_free(handle)
 
 
 def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, problem=PROBLEM_NONE,

@@ -19,7 +19,7 @@ STATUS_NAMES = {0: "RKDG_OK", 1: "RKDG_ERR_ARG", 2: "RKDG_ERR_HIP", 3: "RKDG_ERR
 # rkdg_precision
 RKDG_FP64_PLAIN, RKDG_FP32_BULK, RKDG_FP64_FAST = 0, 1, 2
 # boundary codes
-BC_LAND, BC_TIDE, BC_FLOW, BC_FUNCTION = 0, 1, 2, 3
+BC_LAND, BC_TIDE, BC_FLOW, BC_FUNCTION, BC_LEVEE = 0, 1, 2, 3, 129
 # problem function sets
 PROBLEM_NONE, PROBLEM_MANUFACTURED, PROBLEM_PARABOLIC_BOWL, PROBLEM_DAM_BREAK = 0, 1, 2, 3
 FRICTION_NONE, FRICTION_CHEZY, FRICTION_MANNING = 0, 1, 2
@@ -37,6 +37,7 @@ class MeshDesc(C.Structure):
         ("n_constituents", C.c_int32),
         ("frequency", _dp), ("forcing_fact", _dp), ("equilib_arg", _dp),
         ("node_amplitude", _dp), ("node_phase", _dp), ("node_manning_n", _dp),
+        ("n_levee_pairs", C.c_int32), ("levee_nodes", _ip), ("levee_param", _dp),
     ]
 
 
@@ -96,6 +97,7 @@ SIGNATURES = {
     "rkdg_ssprk_coeffs": (C.c_int, [C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "rkdg_mesh_generate": (C.c_int, [C.POINTER(QuadMeshSpec), C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
     "rkdg_mesh_free": (None, [C.c_void_p]),
+    "rkdg_mesh_read_adcirc": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
 }
 
 _lib = None

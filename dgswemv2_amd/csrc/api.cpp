@@ -20,8 +20,15 @@
 #include "model.hpp"
 
 namespace {
-
 thread_local std::string g_create_error;
+}  // namespace
+
+namespace rkdg {
+// Sets the message rkdg_create_error() returns (also used by the mesh reader).
+void set_create_error(const std::string& msg) { g_create_error = msg; }
+}  // namespace rkdg
+
+namespace {
 
 struct EngineBase {
     virtual ~EngineBase() = default;
@@ -47,7 +54,7 @@ struct Engine final : EngineBase {
     std::vector<Real*> q;      // reference state slots 0..nstages
     std::vector<Real*> sol;    // per stage, only where a later stage reads it
     Real *bath = nullptr, *geom = nullptr, *egeom = nullptr, *vxy = nullptr, *gmn2 = nullptr;
-    Real *famp = nullptr, *fphase = nullptr;
+    Real *famp = nullptr, *fphase = nullptr, *levee_gp = nullptr;
     int *nb = nullptr, *edge = nullptr, *eforce = nullptr, *neighbor = nullptr;
     uint32_t* meta = nullptr;
     uint8_t* wet[2] = {nullptr, nullptr};
@@ -120,7 +127,8 @@ struct Engine final : EngineBase {
             if (needed && !alloc(sol[s], S)) return RKDG_ERR_HIP;
         }
         if (!upload(bath, M.bath) || !upload(geom, M.geom) || !upload(egeom, M.egeom) || !upload(vxy, M.vxy) ||
-            !upload(gmn2, M.gmn2) || !upload(famp, M.famp) || !upload(fphase, M.fphase) || !upload(nb, M.nb) ||
+            !upload(gmn2, M.gmn2) || !upload(famp, M.famp) || !upload(fphase, M.fphase) ||
+            !upload(levee_gp, M.levee_gp) || !upload(nb, M.nb) ||
             !upload(edge, M.edge) || !upload(eforce, M.eforce) || !upload(meta, M.meta) ||
             !upload(neighbor, M.neighbor) || !upload(bath_vrtx, M.bath_vrtx))
             return RKDG_ERR_HIP;
@@ -178,6 +186,7 @@ struct Engine final : EngineBase {
         A.bath = bath, A.geom = geom, A.nb = nb, A.edge = edge, A.meta = meta, A.egeom = egeom;
         A.wet_in = wet[wcur], A.wet_out = wet[wcur ^ 1];
         A.bath_vrtx = bath_vrtx, A.gmn2 = gmn2, A.vxy = vxy, A.eforce = eforce, A.famp = famp, A.fphase = fphase;
+        A.levee_gp = levee_gp;
         A.ncon = M.ncon;
         for (int c = 0; c < M.ncon; ++c)
             A.freq[c] = (Real)M.freq[c], A.ffact[c] = (Real)M.ffact[c], A.eqarg[c] = (Real)M.eqarg[c];

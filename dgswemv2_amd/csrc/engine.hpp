@@ -66,7 +66,8 @@ struct StageArgs {
     const double* bath_vrtx; // [3][ne] wetting-drying vertex depths (FP64 in every form)
     const Real* gmn2;        // [ne] g * mean(n)^2 (Manning)
     const Real* vxy;         // [6][ne] vertex x0 x1 x2 y0 y1 y2
-    const int* eforce;       // [nedge] forcing record or -1
+    const int* eforce;       // [nedge] tide/flow forcing record, levee record, or -1
+    const Real* levee_gp;    // [nlevee][3][NE] barrier height, C_subcritical, C_supercritical at in-side points
     const Real* famp;        // [nforced][ncon][NE]
     const Real* fphase;      // [nforced][ncon][NE]
     int ncon;
@@ -254,6 +255,48 @@ RKDG_HD inline void interface_flux(Real grav, const Real* qi, const Real* qx, Re
     }
     const Real* src = want_in ? Fi : Fe;
     F[0] = src[0], F[1] = src[1], F[2] = src[2];
+}
+
+// Levee (internal barrier) flux of one side, ISP::Levee::ComputeFlux
+// (interface_specializations/rkdg_swe_is_levee.hpp:62-174) seen from that side.  qo: this
+// side's trace at its point, qnb: the other side's at the mirrored point, bo: this side's
+// bathymetry, (nx, ny): the in-side normal (the ex side's is its negation), H/Cs/Cp: barrier
+// height and weir coefficients at the in-side point.  Both sides below the crest, or level
+// within 1 cm: reflective (Land) exterior state.  Otherwise the higher side overtops: the
+// exterior momentum is the weir discharge (subcritical when the lower side's head exceeds
+// 2/3 of the higher's) along the overtopping side's normal; a dry receiving side sees no
+// gravity in its LLF flux.
+template <typename Real>
+RKDG_HD inline void levee_flux(Real grav, bool is_in, const Real* qo, const Real* qnb, Real bo, Real nx, Real ny,
+                               Real H, Real Cs, Real Cp, bool wet_own, Real* F) {
+    using std::abs;
+    using std::sqrt;
+    const Real tol = Real(0.01);  // H_tolerance
+    const Real ze_in = is_in ? qo[0] : qnb[0], ze_ex = is_in ? qnb[0] : qo[0];
+    const Real h_in = ze_in - H, h_ex = ze_ex - H;
+    const Real onx = is_in ? nx : -nx, ony = is_in ? ny : -ny;  // this side's normal
+    Real qe[3], gravity = grav;
+    if ((h_in <= tol && h_ex <= tol) || abs(h_in - h_ex) <= tol) {  // reflective on both sides
+        const Real tx = -ony, ty = onx;
+        const Real qn_ex = -(qo[1] * onx + qo[2] * ony);
+        const Real qt_ex = qo[1] * tx + qo[2] * ty;
+        qe[0] = qo[0], qe[1] = qn_ex * onx + qt_ex * tx, qe[2] = qn_ex * ony + qt_ex * ty;
+    } else {
+        const bool from_in = h_in > h_ex;  // overtopping from in to ex, else from ex to in
+        const Real hu = from_in ? h_in : h_ex, hd = from_in ? h_ex : h_in;  // upstream / downstream heads
+        const Real n_x = from_in ? nx : -nx, n_y = from_in ? ny : -ny;
+        const Real t_x = -n_y, t_y = n_x;
+        Real qn;
+        if (hd > Real(2.0) * hu / Real(3.0))  // subcritical
+            qn = Cs * hd * sqrt(Real(2.0) * grav * (hu - hd));
+        else  // supercritical
+            qn = Cp * (Real(2.0) * hu / Real(3.0)) * sqrt(grav * (Real(2.0) * hu / Real(3.0)));
+        const Real qt = Real(0.0);
+        qe[0] = qo[0], qe[1] = qn * n_x + qt * t_x, qe[2] = qn * n_y + qt * t_y;
+        const bool receiving = from_in ? !is_in : is_in;
+        if (receiving && !wet_own) gravity = Real(0.0);
+    }
+    llf(gravity, qo, qe, bo, onx, ony, F);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -590,6 +633,7 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
             bo[g] = acc;
         }
         Real F[3][NE];
+        const bool lev = (meta >> (12 + s)) & 1;
         if (interior) {
             const bool is_in = e < nbv;
             const Real* pn = FT.phie + (size_t)fn * ND * NE;
@@ -617,20 +661,33 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
                     bn[g] = acc;
                 }
             }
+            if (lev) {  // levee interface (ISP::Levee)
+                const Real* lp = A.levee_gp + (size_t)A.eforce[ed] * 3 * NE;
 #pragma unroll
-            for (int g = 0; g < NE; ++g) {
-                const int gm = NE - 1 - g;  // mirrored point on the other side
-                Real Fg[3];
-                if (is_in) {
-                    const Real qi[3] = {qo[0][g], qo[1][g], qo[2][g]};
-                    const Real qx[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
-                    interface_flux<Real, WD>(grav, qi, qx, bo[g], bn[gm], nx, ny, wet, wet_nb, true, Fg);
-                } else {
-                    const Real qi[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
-                    const Real qx[3] = {qo[0][g], qo[1][g], qo[2][g]};
-                    interface_flux<Real, WD>(grav, qi, qx, bn[gm], bo[g], nx, ny, wet_nb, wet, false, Fg);
+                for (int g = 0; g < NE; ++g) {
+                    const int gm = NE - 1 - g, gi = is_in ? g : gm;  // gi: the in-side point
+                    const Real qo_g[3] = {qo[0][g], qo[1][g], qo[2][g]};
+                    const Real qn_g[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
+                    Real Fg[3];
+                    levee_flux<Real>(grav, is_in, qo_g, qn_g, bo[g], nx, ny, lp[gi], lp[NE + gi], lp[2 * NE + gi], wet, Fg);
+                    F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
                 }
-                F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
+            } else {
+#pragma unroll
+                for (int g = 0; g < NE; ++g) {
+                    const int gm = NE - 1 - g;  // mirrored point on the other side
+                    Real Fg[3];
+                    if (is_in) {
+                        const Real qi[3] = {qo[0][g], qo[1][g], qo[2][g]};
+                        const Real qx[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
+                        interface_flux<Real, WD>(grav, qi, qx, bo[g], bn[gm], nx, ny, wet, wet_nb, true, Fg);
+                    } else {
+                        const Real qi[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
+                        const Real qx[3] = {qo[0][g], qo[1][g], qo[2][g]};
+                        interface_flux<Real, WD>(grav, qi, qx, bn[gm], bo[g], nx, ny, wet_nb, wet, false, Fg);
+                    }
+                    F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
+                }
             }
         } else {
             const int bc = -1 - nbv;
@@ -996,7 +1053,7 @@ RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phi
         Real qn_next[3][ND], bn_next[ND];
         if (kAhead && f < 2) load_nb(f + 1, qn_next, bn_next);
         if (!kAhead) load_nb(f, qn_m, bn_m);
-        if (nbv[f] >= 0 && (wet || wet_nb[f])) {
+        if (nbv[f] >= 0 && !((meta >> (12 + f)) & 1) && (wet || wet_nb[f])) {
             const Real fnx = kAhead ? nx[f] : ldc(A.egeom, A.nedge, 0, ed[f]);
             const Real fny = kAhead ? ny[f] : ldc(A.egeom, A.nedge, 1, ed[f]);
             const Real fsJ = kAhead ? sJ[f] : ldc(A.egeom, A.nedge, 2, ed[f]);
@@ -1054,6 +1111,61 @@ RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phi
                 for (int k = 0; k < 3; ++k) qn_m[k][d] = qn_next[k][d];
                 bn_m[d] = bn_next[d];
             }
+        }
+    }
+
+    // ---- levee faces (internal barriers; rare: rolled loop, tables from LDS).  The weir state is
+    // evaluated on the plain traces and the hydrostatic reference subtracted afterwards.
+    if (meta & (7u << 12)) {
+#pragma unroll 1
+        for (int f = 0; f < 3; ++f) {
+            if (!((meta >> (12 + f)) & 1)) continue;
+            const int nbe = f == 0 ? nbv[0] : (f == 1 ? nbv[1] : nbv[2]);
+            const bool wnb = f == 0 ? wet_nb[0] : (f == 1 ? wet_nb[1] : wet_nb[2]);
+            if (!(wet || wnb)) continue;
+            const int fed = f == 0 ? ed[0] : (f == 1 ? ed[1] : ed[2]);
+            const int fn = (meta >> (4 * f + 2)) & 3;
+            const Real fnx = ldc(A.egeom, A.nedge, 0, fed), fny = ldc(A.egeom, A.nedge, 1, fed);
+            const Real fsJ = ldc(A.egeom, A.nedge, 2, fed);
+            const Real* pf = phie_lds + f * ND * NE;
+            const Real* pn = phie_lds + fn * ND * NE;
+            const Real* pw = phiew_lds + f * ND * NE;
+            Real zn_m[ND];
+#pragma unroll
+            for (int d = 0; d < ND; ++d) zn_m[d] = ldc(A.q_s, ne, d, nbe);
+            const bool is_in = e < nbe;
+            const Real* lp = A.levee_gp + (size_t)A.eforce[fed] * 3 * NE;
+            const Real onx = is_in ? fnx : -fnx, ony = is_in ? fny : -fny;
+            Real F[3][NE];
+#pragma unroll
+            for (int g = 0; g < NE; ++g) {
+                const int gm = NE - 1 - g, gi = is_in ? g : gm;
+                Real zo = Real(0), o1 = Real(0), o2 = Real(0), bo = Real(0), zn = Real(0);
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const Real w = pf[d * NE + g];
+                    if (d > 0) zo = fmaR(q[0][d], w, zo);
+                    o1 = fmaR(q[1][d], w, o1), o2 = fmaR(q[2][d], w, o2), bo = fmaR(bath[d], w, bo);
+                    zn = fmaR(zn_m[d], pn[d * NE + gm], zn);
+                }
+                const Real own[3] = {zc + zo, o1, o2}, nbs[3] = {zn, Real(0), Real(0)};
+                Real Fg[3];
+                levee_flux<Real>(grav, is_in, own, nbs, bo, fnx, fny, lp[gi], lp[NE + gi], lp[2 * NE + gi], wet, Fg);
+                const Real rref = grav * zr * (Real(0.5) * zr + bo);
+                F[0][g] = Fg[0], F[1][g] = Fg[1] - rref * onx, F[2][g] = Fg[2] - rref * ony;
+            }
+            const Real scale = fsJ * rdet;
+#pragma unroll
+            for (int g = 0; g < NE; ++g) F[0][g] *= scale, F[1][g] *= scale, F[2][g] *= scale;
+#pragma unroll
+            for (int d = 0; d < ND; ++d)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    Real r = rhs[k][d];
+#pragma unroll
+                    for (int g = 0; g < NE; ++g) r = fmac_tab(F[k][g], -pw[d * NE + g], r);
+                    rhs[k][d] = r;
+                }
         }
     }
 

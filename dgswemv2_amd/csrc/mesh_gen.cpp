@@ -13,14 +13,8 @@
 #include <vector>
 
 #include "../../include/rkdg.h"
+#include "mesh_owner.hpp"
 
-struct rkdg_mesh {
-    std::vector<double> xyz;
-    std::vector<int32_t> elt;
-    std::vector<int32_t> bedge;
-    std::vector<uint8_t> btype;
-    std::vector<double> freq, ff, eq, amp, phase, manning;
-};
 
 namespace {
 
@@ -170,20 +164,7 @@ extern "C" rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* sp, rkdg_me
     if (sp->manning_n > 0) M->manning.assign(nn, sp->manning_n);
 
     *desc = rkdg_mesh_desc{};
-    desc->n_nodes = (int32_t)nn;
-    desc->n_elements = 2 * m * n;
-    desc->node_xyz = M->xyz.data();
-    desc->elt_nodes = M->elt.data();
-    desc->n_bound_edges = (int32_t)M->btype.size();
-    desc->bound_edge_nodes = M->bedge.data();
-    desc->bound_edge_type = M->btype.data();
-    desc->n_constituents = (int32_t)M->freq.size();
-    desc->frequency = M->freq.empty() ? nullptr : M->freq.data();
-    desc->forcing_fact = M->ff.empty() ? nullptr : M->ff.data();
-    desc->equilib_arg = M->eq.empty() ? nullptr : M->eq.data();
-    desc->node_amplitude = M->amp.empty() ? nullptr : M->amp.data();
-    desc->node_phase = M->phase.empty() ? nullptr : M->phase.data();
-    desc->node_manning_n = M->manning.empty() ? nullptr : M->manning.data();
+    M->fill(desc);
     *out = M;
     return RKDG_OK;
 }

@@ -150,6 +150,8 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
     }
     std::vector<int> nbr(3 * (size_t)ne, kNoNeighbour), nbr_face(3 * (size_t)ne, 0);
     std::vector<int> btype(3 * (size_t)ne, -1);  // BC code of boundary faces
+    std::vector<uint8_t> levee(3 * (size_t)ne, 0);  // face is a levee (barrier) interface
+    std::map<std::pair<int, int>, int> levee_pair;  // (min, max) node pair -> levee_nodes row
     {
         std::unordered_map<uint64_t, std::array<int, 4>> dict;
         dict.reserve((size_t)ne * 2);
@@ -174,10 +176,38 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
                 const int a = (int)(kv.first >> 32), b = (int)(kv.first & 0xffffffffu);
                 auto it = btype_of.find({a, b});
                 if (it == btype_of.end()) return "boundary edge without a boundary type";
-                if (it->second > RKDG_BC_FUNCTION) return "unsupported boundary type";
+                if (it->second > RKDG_BC_FUNCTION && it->second != RKDG_BC_LEVEE) return "unsupported boundary type";
                 btype[3 * v[0] + v[1]] = it->second;
             }
         }
+        // levee faces: the element across the barrier owns the partner edge, whose nodes are the
+        // barrier partners of this edge's nodes (AdcircFormat::get_internal_node_pair,
+        // adcirc_format.cpp:260-310; MeshMetaData, mesh_metadata.cpp:74-80)
+        std::unordered_map<int, int> partner;
+        for (int i = 0; i < md.n_levee_pairs; ++i) {
+            const int f = md.levee_nodes[2 * i], b = md.levee_nodes[2 * i + 1];
+            if (f < 0 || f >= md.n_nodes || b < 0 || b >= md.n_nodes) return "levee node out of range";
+            if (partner.count(f) || partner.count(b)) return "levee node in more than one pair";
+            partner[f] = b, partner[b] = f;
+            levee_pair[{std::min(f, b), std::max(f, b)}] = i;
+        }
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < 3; ++k) {
+                if (btype[3 * e + k] != RKDG_BC_LEVEE) continue;
+                const int a = md.elt_nodes[3 * e + (k + 1) % 3], b = md.elt_nodes[3 * e + (k + 2) % 3];
+                auto pa = partner.find(a), pb = partner.find(b);
+                if (pa == partner.end() || pb == partner.end()) return "levee edge without barrier node pairs";
+                const uint32_t x = pa->second, y = pb->second;
+                auto it = dict.find((uint64_t)std::min(x, y) << 32 | std::max(x, y));
+                if (it == dict.end() || it->second[2] >= 0) return "levee edge without a single partner element";
+                nbr[3 * e + k] = it->second[0], nbr_face[3 * e + k] = it->second[1];
+                levee[3 * e + k] = 1;
+                btype[3 * e + k] = -1;
+            }
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < 3; ++k)
+                if (levee[3 * e + k] && (nbr[3 * nbr[3 * e + k] + nbr_face[3 * e + k]] != e))
+                    return "levee partner edges do not pair up";
     }
 
     // ---- element geometry (Element::Initialize, element.hpp:166-272)
@@ -207,21 +237,47 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
         for (int k = 0; k < 3; ++k) M.vxy[(size_t)k * ne + e] = T.x[k], M.vxy[(size_t)(3 + k) * ne + e] = T.y[k];
     }
 
-    // ---- edge records: interfaces in (in, ex) order, then boundaries by (type, element, face)
+    // ---- edge records: interfaces in (in, ex) order, then boundaries by (type, element, face).
+    // Interfaces, levees included, carry the in-side normal and surface Jacobian, which the ex
+    // side uses negated (interface.hpp:206-217).  A levee edge also has a record of barrier
+    // height and weir coefficients at the in-side Gauss points (ISP::Levee::Initialize,
+    // rkdg_swe_is_levee.hpp:50-60: nodal values times GetBoundaryPsi).
     std::vector<int> face_edge(3 * (size_t)ne, -1);
     std::vector<double> eg[3];
+    M.eforce.clear();
     for (int e = 0; e < ne; ++e) {
         std::vector<std::pair<int, int>> up;
         for (int k = 0; k < 3; ++k)
             if (nbr[3 * e + k] > e) up.push_back({nbr[3 * e + k], k});
         std::sort(up.begin(), up.end());
         for (auto& pr : up) {
-            const int id = (int)eg[0].size();
+            const int id = (int)eg[0].size(), k = pr.second;
             double nx, ny;
-            tri[e].normal(pr.second, nx, ny);
-            eg[0].push_back(nx), eg[1].push_back(ny), eg[2].push_back(tri[e].surfaceJ(pr.second));
-            face_edge[3 * e + pr.second] = id;
-            face_edge[3 * pr.first + nbr_face[3 * e + pr.second]] = id;
+            tri[e].normal(k, nx, ny);
+            eg[0].push_back(nx), eg[1].push_back(ny), eg[2].push_back(tri[e].surfaceJ(k));
+            face_edge[3 * e + k] = id;
+            face_edge[3 * pr.first + nbr_face[3 * e + k]] = id;
+            int rec = -1;
+            if (levee[3 * e + k]) {
+                // in-side node i pairs with the ex-side face node 1 - i (swe_pre_create_intface.hpp:50-69)
+                const int x = pr.first, kx = nbr_face[3 * e + k];
+                const int in_n[2] = {md.elt_nodes[3 * e + (k + 1) % 3], md.elt_nodes[3 * e + (k + 2) % 3]};
+                const int ex_n[2] = {md.elt_nodes[3 * x + (kx + 1) % 3], md.elt_nodes[3 * x + (kx + 2) % 3]};
+                double par[2][3];
+                for (int i = 0; i < 2; ++i) {
+                    const int a = in_n[i], b = ex_n[1 - i];
+                    auto it = levee_pair.find({std::min(a, b), std::max(a, b)});
+                    if (it == levee_pair.end()) return "unable to find levee data for a barrier edge";
+                    for (int c = 0; c < 3; ++c) par[i][c] = md.levee_param[3 * (size_t)it->second + c];
+                }
+                rec = (int)(M.levee_gp.size() / (3 * (size_t)nge));
+                for (int c = 0; c < 3; ++c)
+                    for (int g = 0; g < nge; ++g) {
+                        const double z = M.master.erule.z[g];
+                        M.levee_gp.push_back(par[0][c] * ((1 - z) / 2) + par[1][c] * ((1 + z) / 2));
+                    }
+            }
+            M.eforce.push_back(rec);
         }
     }
     M.nintf = (int)eg[0].size();
@@ -237,7 +293,6 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
         M.ffact.push_back(md.forcing_fact[c]);
         M.eqarg.push_back(md.equilib_arg[c]);
     }
-    M.eforce.assign(M.nintf, -1);
     for (auto& b : bedges) {
         const int id = (int)eg[0].size(), e = b[1], k = b[2];
         double nx, ny;
@@ -276,12 +331,14 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
     M.neighbor.assign(3 * (size_t)ne, -1);
     M.sl_face_slot.assign(3 * (size_t)ne, 0);
     for (int e = 0; e < ne; ++e) {
-        std::array<std::array<int, 3>, 3> key;  // (class, order key, face)
+        // (class, order key, face): internal interfaces by neighbour ID, then levee interfaces by
+        // neighbour ID (the interface vectors run Internal then Levee), then boundaries
+        std::array<std::array<int, 3>, 3> key;
         for (int k = 0; k < 3; ++k) {
             if (nbr[3 * e + k] >= 0)
-                key[k] = {0, nbr[3 * e + k], k};
+                key[k] = {levee[3 * e + k] ? 1 : 0, nbr[3 * e + k], k};
             else
-                key[k] = {1, btype[3 * e + k], k};
+                key[k] = {2, btype[3 * e + k], k};
             M.neighbor[(size_t)k * ne + e] = nbr[3 * e + k];
         }
         if (reference_face_order) std::sort(key.begin(), key.end());
@@ -292,6 +349,7 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
             M.edge[(size_t)s * ne + e] = face_edge[3 * e + k];
             meta |= (uint32_t)k << (4 * s);
             meta |= (uint32_t)(nbr[3 * e + k] >= 0 ? nbr_face[3 * e + k] : 0) << (4 * s + 2);
+            meta |= (uint32_t)levee[3 * e + k] << (12 + s);
             M.sl_face_slot[(size_t)k * ne + e] = s;
         }
         M.meta[e] = meta;

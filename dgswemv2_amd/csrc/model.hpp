@@ -41,7 +41,8 @@ struct Model {
     bool closed_forms = false;      // manufactured source or function boundaries present
     std::vector<int> nb;            // [3][ne] per slot: neighbour, or -1-bc
     std::vector<int> edge;          // [3][ne] per slot: edge record
-    std::vector<uint32_t> meta;     // [ne] own/neighbour local face per slot
+    std::vector<uint32_t> meta;     // [ne] per slot s: own face bits 4s..4s+1, neighbour face 4s+2..4s+3,
+                                    // levee flag bit 12+s
     std::vector<int> neighbor;      // [3][ne] by local face: neighbour or -1 (limiter / dilation)
     // limiter geometry (swe_pre_init_data.hpp:252-326), [k][ne]
     std::vector<double> sl_median;  // [6][ne] mx0 my0 mx1 my1 mx2 my2
@@ -54,7 +55,8 @@ struct Model {
 
     // per edge record: interfaces (in-side geometry) then boundaries (own geometry)
     std::vector<double> egeom;      // [3][nedge] nx ny sJ
-    std::vector<int> eforce;        // [nedge] forcing record or -1
+    std::vector<int> eforce;        // [nedge] tide/flow forcing record, levee record, or -1
+    std::vector<double> levee_gp;   // [nlevee][3][ngpe] H_barrier, C_subcritical, C_supercritical at in-side points
     int ncon = 0;
     std::vector<double> freq, ffact, eqarg;
     std::vector<double> famp, fphase;  // [nforced][ncon][ngpe]

@@ -48,7 +48,8 @@ enum {
     RKDG_BC_LAND = 0,
     RKDG_BC_TIDE = 1,
     RKDG_BC_FLOW = 2,
-    RKDG_BC_FUNCTION = 3
+    RKDG_BC_FUNCTION = 3,
+    RKDG_BC_LEVEE = 129     /* internal barrier side (INTERNAL + 1, utilities/edge_types.hpp:4) */
 };
 
 /* Closed-form problem function sets (reference: compile-time ic_q/source_q/true_q files,
@@ -86,6 +87,14 @@ typedef struct rkdg_mesh_desc {
     const double* node_amplitude;    /* [n_nodes*n_constituents] or NULL */
     const double* node_phase;        /* [n_nodes*n_constituents] degrees, or NULL */
     const double* node_manning_n;    /* [n_nodes] or NULL */
+    /* Internal barriers (levees).  Both node strings of a barrier are boundary edges of code
+     * RKDG_BC_LEVEE; pair i joins front node levee_nodes[2i] with back node levee_nodes[2i+1]
+     * (reference: IBTYPE %10 == 4 NBVV/IBCONN in the .14, adcirc_format.cpp:89-102) and
+     * carries the barrier height and the sub-/supercritical weir coefficients at that node
+     * pair (the .bcis levee block, swe_inputs.cpp:298-310). */
+    int32_t n_levee_pairs;
+    const int32_t* levee_nodes;      /* [n_levee_pairs*2] front, back */
+    const double* levee_param;       /* [n_levee_pairs*3] H_barrier, C_subcritical, C_supercritical */
 } rkdg_mesh_desc;
 
 /* Run parameters (reference: InputParameters + SWE::Inputs, preprocessor/input_parameters.hpp:96-261,
@@ -216,6 +225,19 @@ typedef struct rkdg_quad_mesh_spec {
 
 typedef struct rkdg_mesh rkdg_mesh; /* owns the arrays a rkdg_mesh_desc points into */
 rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* spec, rkdg_mesh** out, rkdg_mesh_desc* desc);
+
+/* Reads an ADCIRC mesh (.14) and, if bcis_path is not NULL, its boundary-condition /
+ * interface-specialisation data (.bcis) into a mesh description.  Replaces AdcircFormat
+ * (preprocessor/ADCIRC_reader/adcirc_format.cpp:3-130) with its boundary typing
+ * AdcircFormat::get_ibtype / get_internal_node_pair (:214-336) and MeshMetaData
+ * (preprocessor/mesh_metadata.cpp:3-85), and SWE::Inputs::read_bcis (swe_inputs.cpp:212-312).
+ * Node and element IDs become their rank among the file's IDs (an order-preserving relabel).
+ * Tide and flow blocks must share one constituent list.  Errors: RKDG_ERR_ARG with the
+ * message in rkdg_create_error(). */
+rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char* bcis_path, rkdg_mesh** out,
+                                  rkdg_mesh_desc* desc);
+
+/* Frees a mesh from rkdg_mesh_generate or rkdg_mesh_read_adcirc. */
 void rkdg_mesh_free(rkdg_mesh* mesh);
 
 #ifdef __cplusplus

@@ -521,7 +521,7 @@ struct Element {
     unsigned id;
     unsigned node_ID[3];
     unsigned neighbor_ID[3];
-    unsigned char btype[3];  // 0..3 BC codes, 128 internal
+    unsigned char btype[3];  // 0..3 BC codes, 128 internal, 129 levee
     Shape shape;
     Mat dphi_gp[2];          // ndof x ngp (element, global derivative)
     std::vector<double> int_fact;
@@ -570,6 +570,8 @@ struct EdgeSide {
 
 struct Interface {
     EdgeSide in, ex;
+    bool levee = false;                  // ISP::Levee instead of ISP::Internal
+    std::vector<double> H_bar, C_sub, C_sup;  // per in-side gp (ISP::Levee::Initialize)
 };
 
 struct Boundary {
@@ -694,6 +696,34 @@ static Sim* create(const rkdg_mesh_desc* md, const rkdg_params* prm) {
             S->elts[v[0]].btype[v[1]] = it->second;
         }
     }
+    // levee faces: the neighbour owns the edge between the barrier partners of this edge's nodes
+    // (get_internal_node_pair, adcirc_format.cpp:260-310; mesh_metadata.cpp:74-80)
+    std::map<unsigned, unsigned> partner;
+    std::map<std::pair<unsigned, unsigned>, std::array<double, 3>> levee_data;  // SWE::Inputs::levee_is_data
+    for (int i = 0; i < md->n_levee_pairs; ++i) {
+        unsigned f = md->levee_nodes[2 * i], b = md->levee_nodes[2 * i + 1];
+        partner[f] = b, partner[b] = f;
+        levee_data[{f, b}] = {md->levee_param[3 * i], md->levee_param[3 * i + 1], md->levee_param[3 * i + 2]};
+    }
+    for (unsigned e = 0; e < ne; ++e)
+        for (unsigned k = 0; k < 3; ++k) {
+            Element& E = S->elts[e];
+            if (E.btype[k] != RKDG_BC_LEVEE) continue;
+            unsigned a = E.node_ID[(k + 1) % 3], b = E.node_ID[(k + 2) % 3];
+            if (!partner.count(a) || !partner.count(b)) {
+                g_err = "levee edge without barrier node pairs";
+                delete S;
+                return nullptr;
+            }
+            unsigned x = partner[a], y = partner[b];
+            auto it = dict.find((uint64_t)std::min(x, y) << 32 | std::max(x, y));
+            if (it == dict.end()) {
+                g_err = "levee edge without a partner element";
+                delete S;
+                return nullptr;
+            }
+            E.neighbor_ID[k] = it->second[0];
+        }
 
     // elements (Element::Initialize, element.hpp:166-272)
     Mat psi_gp = Shape::psi(m.rule.z);
@@ -768,6 +798,50 @@ static Sim* create(const rkdg_mesh_desc* md, const rkdg_params* prm) {
             auto nrm = E.shape.normal(pr.second);
             make_side(*S, I.in, E, pr.second, erule, sJ, nrm, false);
             make_side(*S, I.ex, X, bx, erule, sJ, nrm, true);
+            S->intfaces.push_back(std::move(I));
+        }
+    }
+    // levee interfaces after all internal ones (the interface tuple is Internal, Levee;
+    // swe_pre_create_intface.hpp:37-76), again keyed (ID, neighbour)
+    for (unsigned e = 0; e < ne; ++e) {
+        Element& E = S->elts[e];
+        std::vector<std::pair<unsigned, unsigned>> nb;
+        for (unsigned k = 0; k < 3; ++k)
+            if (E.btype[k] == RKDG_BC_LEVEE && E.neighbor_ID[k] > e) nb.push_back({E.neighbor_ID[k], k});
+        std::sort(nb.begin(), nb.end());
+        for (auto& pr : nb) {
+            Element& X = S->elts[pr.first];
+            unsigned bx = 0;
+            for (unsigned k = 0; k < 3; ++k)
+                if (X.btype[k] == RKDG_BC_LEVEE && X.neighbor_ID[k] == e) bx = k;
+            Interface I;
+            I.levee = true;
+            double sJ = E.shape.surfaceJ(pr.second);
+            auto nrm = E.shape.normal(pr.second);
+            make_side(*S, I.in, E, pr.second, erule, sJ, nrm, false);
+            make_side(*S, I.ex, X, bx, erule, sJ, nrm, true);
+            // levee data per in-side node, keyed {in node, ex node reversed} or swapped
+            std::array<double, 3> nodal[2];
+            for (unsigned node = 0; node < 2; ++node) {
+                std::pair<unsigned, unsigned> key{I.in.node_ID[node], I.ex.node_ID[2 - node - 1]};
+                std::pair<unsigned, unsigned> swp{key.second, key.first};
+                if (levee_data.count(key))
+                    nodal[node] = levee_data[key];
+                else if (levee_data.count(swp))
+                    nodal[node] = levee_data[swp];
+                else {
+                    g_err = "unable to find levee data";
+                    delete S;
+                    return nullptr;
+                }
+            }
+            size_t ngpe = I.in.int_fact.size();
+            I.H_bar.resize(ngpe), I.C_sub.resize(ngpe), I.C_sup.resize(ngpe);
+            for (size_t g = 0; g < ngpe; ++g) {  // ComputeBoundaryNodalUgpIN: nodal * psi_bound_gp_in
+                I.H_bar[g] = nodal[0][0] * I.in.psi_bound(0, (int)g) + nodal[1][0] * I.in.psi_bound(1, (int)g);
+                I.C_sub[g] = nodal[0][1] * I.in.psi_bound(0, (int)g) + nodal[1][1] * I.in.psi_bound(1, (int)g);
+                I.C_sup[g] = nodal[0][2] * I.in.psi_bound(0, (int)g) + nodal[1][2] * I.in.psi_bound(1, (int)g);
+            }
             S->intfaces.push_back(std::move(I));
         }
     }
@@ -1002,11 +1076,70 @@ static void subtract_lift(Sim& S, EdgeSide& s) {
         for (int d = 0; d < T.c; ++d) rhs(k, d) = rhs(k, d) - T(k, d);
 }
 
+// Land exterior state (BC::Land::GetEX, rkdg_swe_bc_land.hpp:40-66).
+static void land_ex(const double* n, const double* q, double* qe) {
+    double tx = -n[1], ty = n[0];
+    double qn_ex = -(q[1] * n[0] + q[2] * n[1]);
+    double qt_ex = q[1] * tx + q[2] * ty;
+    qe[0] = q[0], qe[1] = qn_ex * n[0] + qt_ex * tx, qe[2] = qn_ex * n[1] + qt_ex * ty;
+}
+
+static void levee_kernel_flux(Sim& S, Interface& I, bool wet_in, bool wet_ex) {  // rkdg_swe_is_levee.hpp:62-174
+    const double H_tolerance = 0.01, G = S.prm.g;
+    unsigned ngp = (unsigned)I.in.int_fact.size();
+    for (unsigned gp = 0; gp < ngp; ++gp) {
+        unsigned gp_ex = ngp - gp - 1;
+        double H = I.H_bar[gp], Cs = I.C_sub[gp], Cp = I.C_sup[gp];
+        double gravity_in = G, gravity_ex = G;
+        double q_in[3], q_ex[3], q_in_ex[3], q_ex_ex[3];
+        for (int k = 0; k < 3; ++k) q_in[k] = I.in.q_at_gp(k, gp), q_ex[k] = I.ex.q_at_gp(k, gp_ex);
+        double h_in = q_in[0] - H, h_ex = q_ex[0] - H;
+        const double* n_in = I.in.normal[gp].data();
+        const double* n_ex = I.ex.normal[gp_ex].data();
+        if ((h_in <= H_tolerance && h_ex <= H_tolerance) || std::abs(h_in - h_ex) <= H_tolerance) {
+            land_ex(n_in, q_in, q_in_ex);
+            land_ex(n_ex, q_ex, q_ex_ex);
+        } else if (h_in > h_ex) {  // overtopping from in to ex
+            double n_x = n_in[0], n_y = n_in[1], t_x = -n_y, t_y = n_x, qn, qt;
+            if (h_ex > 2.0 * h_in / 3.0)
+                qn = Cs * h_ex * std::sqrt(2.0 * G * (h_in - h_ex));
+            else
+                qn = Cp * (2.0 * h_in / 3.0) * std::sqrt(G * (2.0 * h_in / 3.0));
+            qt = 0.0;
+            q_in_ex[0] = q_in[0], q_in_ex[1] = qn * n_x + qt * t_x, q_in_ex[2] = qn * n_y + qt * t_y;
+            q_ex_ex[0] = q_ex[0], q_ex_ex[1] = q_in_ex[1], q_ex_ex[2] = q_in_ex[2];
+            if (!wet_ex) gravity_ex = 0.0;
+        } else if (h_in < h_ex) {  // overtopping from ex to in
+            double n_x = n_ex[0], n_y = n_ex[1], t_x = -n_y, t_y = n_x, qn, qt;
+            if (h_in > 2.0 * h_ex / 3.0)
+                qn = Cs * h_in * std::sqrt(2.0 * G * (h_ex - h_in));
+            else
+                qn = Cp * (2.0 * h_ex / 3.0) * std::sqrt(G * (2.0 * h_ex / 3.0));
+            qt = 0.0;
+            q_ex_ex[0] = q_ex[0], q_ex_ex[1] = qn * n_x + qt * t_x, q_ex_ex[2] = qn * n_y + qt * t_y;
+            q_in_ex[0] = q_in[0], q_in_ex[1] = q_ex_ex[1], q_in_ex[2] = q_ex_ex[2];
+            if (!wet_in) gravity_in = 0.0;
+        }
+        double ai[3], ae[3], F[3];
+        for (int k = 0; k < 3; ++k) ai[k] = I.in.aux_at_gp(k, gp), ae[k] = I.ex.aux_at_gp(k, gp_ex);
+        LLF_flux(gravity_in, q_in, q_in_ex, ai, n_in, F);
+        for (int k = 0; k < 3; ++k) I.in.F_hat(k, gp) = F[k];
+        LLF_flux(gravity_ex, q_ex, q_ex_ex, ae, n_ex, F);
+        for (int k = 0; k < 3; ++k) I.ex.F_hat(k, gp_ex) = F[k];
+    }
+}
+
 static void interface_kernel(Sim& S, Interface& I) {  // rkdg_swe_proc_intface.hpp:6-25
     bool wet_in = S.elts[I.in.elt].wet, wet_ex = S.elts[I.ex.elt].wet;
     if (!(wet_in || wet_ex)) return;
     side_trace(S, I.in);
     side_trace(S, I.ex);
+    if (I.levee) {
+        levee_kernel_flux(S, I, wet_in, wet_ex);
+        subtract_lift(S, I.in);
+        subtract_lift(S, I.ex);
+        return;
+    }
     unsigned ngp = (unsigned)I.in.int_fact.size();
     double g = S.prm.g;
     for (unsigned gp = 0; gp < ngp; ++gp) {  // ISP::Internal::ComputeFlux (rkdg_swe_is_internal.hpp:24-94)

@@ -210,3 +210,55 @@ def test_full_size_tidal_fp32_vs_fp64():
     r = relerr(qb, qa)
     banner("full-size tidal fp32 vs fp64 rel L2", r)
     assert max(r) < 1e-4
+
+
+# ---- levee (internal barrier) interfaces: the reference's weir fixture, overtopping from the start
+def _weir_case(p, slope_limiting):
+    from test_levee import WEIR, weir_params
+    from dgswemv2_amd import read_adcirc
+    mesh = read_adcirc(WEIR / "weir.14", WEIR / "weir.bcis")
+    prm = weir_params(p=p, slope_limiting=slope_limiting)
+    o = OracleSim(mesh, prm)
+    c = OracleSim(mesh, make_params(p=p, ic_constant=True, ze0=1.0)).state()["q"][0, 0, 0]
+    yc = mesh.node_xyz[mesh.elt_nodes, 1].mean(axis=1)
+    q = np.zeros_like(o.state()["q"])
+    q[0, 0, :] = np.where(yc < 15000, 0.5, 0.0) * c  # 0.5 m below the barrier: overtops the 0.1 m crest
+    o.set_state(q)
+    return mesh, prm, o, q
+
+
+@pytest.mark.parametrize("p,steps", [(1, 20), (2, 10)])
+def test_weir_levee_plain_vs_oracle(p, steps):
+    mesh, prm, o, q = _weir_case(p, True)
+    banner("weir levee p", p, "ne", mesh.n_elements, "steps", steps)
+    g = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    g.set_state(q)
+    for i in range(steps):
+        o.step(1), g.step(1)
+        so, sg = o.state(), g.state()
+        r = max(relerr(sg["q"], so["q"]))
+        assert r <= TOL64, (i, r)
+        for f in ("wet", "limited", "troubled"):
+            assert np.array_equal(sg[f], so[f]), (i, f)
+    banner("weir levee p", p, "bit-exact", bool(np.array_equal(g.state()["q"], o.state()["q"])))
+
+
+@pytest.mark.parametrize("p", [1, 2])
+def test_weir_levee_fast_forms_vs_oracle(p):
+    # without the limiter (its 1e-12 write-back test turns roundoff into O(1) changes at p = 2);
+    # qx is small here (flow crosses the barrier in y), so it is measured against |(qx, qy)|
+    mesh, prm, o, q = _weir_case(p, False)
+    f64 = Simulation(mesh, prm, RKDG_FP64_FAST)
+    f32 = Simulation(mesh, prm, RKDG_FP32_BULK)
+    f64.set_state(q), f32.set_state(q)
+    o.step(10), f64.step(10), f32.step(10)
+    ref = o.state()["q"]
+    mom = np.linalg.norm(ref[1:])
+
+    def err(s):
+        return [np.linalg.norm(s[0] - ref[0]) / np.linalg.norm(ref[0]), np.linalg.norm(s[1] - ref[1]) / mom,
+                np.linalg.norm(s[2] - ref[2]) / mom]
+    e64, e32 = err(f64.state()["q"]), err(f32.state()["q"])
+    banner("weir fast forms p", p, "fp64-fast", e64, "fp32", e32)
+    assert max(e64) <= TOL64
+    assert max(e32) <= 1e-5
