@@ -17,7 +17,7 @@ from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "read_adcirc", "make_params", "Simulation", "RkdgError", "ndof_of",
+    "Mesh", "quad_mesh", "read_adcirc", "make_params", "Simulation", "Batch", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
@@ -263,3 +263,96 @@ class Simulation:
         n = int(math.ceil(run_time / self.params.dt))
         self.step(n)
         return n
+
+
+class Batch:
+    """An ensemble of independent meshes stepped together on one GPU (rkdg_create_batch).
+    Instances of equal p share one merged element range and one stage launch per stage."""
+
+    def __init__(self, meshes, params, precision=RKDG_FP32_BULK):
+        if len(meshes) != len(params) or not meshes:
+            raise ValueError("one params entry per mesh, at least one mesh")
+        self.lib = N.load_library()
+        self.meshes, self.params, self.precision = list(meshes), list(params), precision
+        n = len(self.meshes)
+        descs = (N.MeshDesc * n)(*[m.desc for m in self.meshes])
+        prms = (N.Params * n)(*self.params)
+        h = C.c_void_p()
+        st = self.lib.rkdg_create_batch(n, descs, prms, precision, C.byref(h))
+        if st != N.RKDG_OK:
+            raise RkdgError(st, self.lib.rkdg_create_error().decode())
+        self.ctx = h
+
+    def __len__(self):
+        return len(self.meshes)
+
+    def _check(self, st):
+        if st != N.RKDG_OK:
+            raise RkdgError(st, self.lib.rkdg_last_error(self.ctx).decode())
+
+    def close(self):
+        if getattr(self, "ctx", None):
+            self.lib.rkdg_destroy(self.ctx)
+            self.ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def step(self, nsteps=1):
+        self._check(self.lib.rkdg_step(self.ctx, nsteps))
+
+    def sync(self):
+        self._check(self.lib.rkdg_sync(self.ctx))
+
+    def state(self, i):
+        m, p = self.meshes[i], self.params[i]
+        ne, nd = m.n_elements, ndof_of(p.p)
+        q = np.empty(3 * nd * ne)
+        w = np.empty(ne, np.uint8)
+        lim = np.empty(3 * ne, np.uint8)
+        tr = np.empty(ne, np.uint8)
+        self._check(self.lib.rkdg_batch_get_state(self.ctx, i, N.dptr(q), N.uptr(w), N.uptr(lim), N.uptr(tr)))
+        return {"q": q.reshape(3, nd, ne), "wet": w, "limited": lim.reshape(3, ne), "troubled": tr}
+
+    def set_state(self, i, q, wet=None):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        w = None if wet is None else np.ascontiguousarray(wet, dtype=np.uint8)
+        self._check(self.lib.rkdg_batch_set_state(self.ctx, i, N.dptr(q), N.uptr(w)))
+
+    def time(self, i):
+        t = C.c_double()
+        step, stage = C.c_int32(), C.c_int32()
+        self._check(self.lib.rkdg_batch_get_time(self.ctx, i, C.byref(t), C.byref(step), C.byref(stage)))
+        return t.value, step.value, stage.value
+
+    def nan_info(self):
+        v = [C.c_int32() for _ in range(4)]
+        self._check(self.lib.rkdg_batch_nan_info(self.ctx, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    @property
+    def schedule(self):
+        sc = N.Schedule()
+        self._check(self.lib.rkdg_batch_get_schedule(self.ctx, C.byref(sc)))
+        return {"block": sc.block, "groups_per_xcd": sc.groups_per_xcd, "graph": sc.graph}
+
+    def set_schedule(self, block=128, groups_per_xcd=0, graph=0):
+        sc = N.Schedule(block, groups_per_xcd, graph)
+        self._check(self.lib.rkdg_batch_set_schedule(self.ctx, C.byref(sc)))
+
+    def autotune(self, steps_per_trial=3):
+        sc = N.Schedule()
+        ms = C.c_double()
+        self._check(self.lib.rkdg_batch_autotune(self.ctx, steps_per_trial, C.byref(sc), C.byref(ms)))
+        return {"block": sc.block, "groups_per_xcd": sc.groups_per_xcd, "graph": sc.graph}, ms.value
+
+    def reset_timers(self):
+        self._check(self.lib.rkdg_reset_timers(self.ctx))
+
+    def report(self):
+        r = N.PerfReport()
+        self._check(self.lib.rkdg_report(self.ctx, C.byref(r)))
+        return {f: getattr(r, f) for f, _ in N.PerfReport._fields_}

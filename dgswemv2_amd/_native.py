@@ -63,6 +63,10 @@ class PerfReport(C.Structure):
     ]
 
 
+class Schedule(C.Structure):
+    _fields_ = [("block", C.c_int32), ("groups_per_xcd", C.c_int32), ("graph", C.c_int32)]
+
+
 class QuadMeshSpec(C.Structure):
     _fields_ = [
         ("x1", C.c_double), ("y1", C.c_double), ("x2", C.c_double), ("y2", C.c_double),
@@ -98,6 +102,16 @@ SIGNATURES = {
     "rkdg_mesh_generate": (C.c_int, [C.POINTER(QuadMeshSpec), C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
     "rkdg_mesh_free": (None, [C.c_void_p]),
     "rkdg_mesh_read_adcirc": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
+    "rkdg_create_batch": (C.c_int, [C.c_int32, C.POINTER(MeshDesc), C.POINTER(Params), C.c_int,
+                                    C.POINTER(C.c_void_p)]),
+    "rkdg_batch_size": (C.c_int32, [C.c_void_p]),
+    "rkdg_batch_get_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _up, _up, _up]),
+    "rkdg_batch_set_state": (C.c_int, [C.c_void_p, C.c_int32, _dp, _up]),
+    "rkdg_batch_get_time": (C.c_int, [C.c_void_p, C.c_int32, _dp, _ip, _ip]),
+    "rkdg_batch_nan_info": (C.c_int, [C.c_void_p, _ip, _ip, _ip, _ip]),
+    "rkdg_batch_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
+    "rkdg_batch_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
+    "rkdg_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Schedule), _dp]),
 }
 
 _lib = None

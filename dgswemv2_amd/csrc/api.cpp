@@ -12,6 +12,8 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <array>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -38,6 +40,14 @@ struct EngineBase {
     virtual rkdg_status get_state(double* q, uint8_t* wet, uint8_t* limited, uint8_t* troubled) = 0;
     virtual rkdg_status nan_key(unsigned long long* key) = 0;
     virtual void collect(rkdg_perf_report* r) = 0;  // adds timed kernel milliseconds, recycles events
+    // ensembles (merged models)
+    virtual rkdg_status inst_get(int i, double* q, uint8_t* wet, uint8_t* lim, uint8_t* trb) = 0;
+    virtual rkdg_status inst_set(int i, const double* q, const uint8_t* wet) = 0;
+    virtual rkdg_status inst_time(int i, double* t, int* step, int* stage) = 0;
+    virtual rkdg_status snapshot(bool restore) = 0;  // save / restore the step-boundary state
+    int block = 128;            // work-items per block (measured best for the stage kernels on 1M elements)
+    int persistent_grid = 0;    // > 0: persistent stage kernel on this many work-groups (ensembles)
+    bool capturing = false;     // inside a graph capture: no host-side event records
     std::string err;
     bool timing = false;
     int64_t stages_timed = 0;
@@ -49,7 +59,6 @@ template <typename Real>
 struct Engine final : EngineBase {
     rkdg::Model& M;
     hipStream_t stream;
-    int block = 128;  // work-items per block (measured best for the stage kernels on 1M elements)
     std::vector<void*> allocs;
     std::vector<Real*> q;      // reference state slots 0..nstages
     std::vector<Real*> sol;    // per stage, only where a later stage reads it
@@ -65,6 +74,12 @@ struct Engine final : EngineBase {
     double *median = nullptr, *alpha = nullptr, *rsq = nullptr, *len = nullptr, *radius = nullptr,
            *bath_bc = nullptr, *baryctr = nullptr;
     uint8_t *set_a = nullptr, *set_b = nullptr, *troubled = nullptr, *limited = nullptr;
+    // ensemble tables (M.ens.n > 0)
+    int *tile_inst_d = nullptr, *inst_end_d = nullptr, *inst_step_d = nullptr, *inst_stage_d = nullptr;
+    double *inst_tr_d = nullptr, *inst_t_d = nullptr, *inst_dt_d = nullptr, *inst_rd_d = nullptr, *drk_d = nullptr;
+    Real* coef_d = nullptr;
+    unsigned long long* step_ctr_d = nullptr;
+    std::vector<void*> snap;  // snapshot buffers (autotuning)
     // timing
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> stage_ev, lim_ev;
@@ -73,12 +88,14 @@ struct Engine final : EngineBase {
 
     hipError_t launch_plain(const rkdg::StageArgs<Real>& A) {
         if constexpr (sizeof(Real) == 8)
-            return rkdg::launch_stage<Real, false>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block);
+            return rkdg::launch_stage<Real, false>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block,
+                                                   persistent_grid);
         else
             return hipErrorInvalidValue;  // FP32 exists only in the fast form
     }
     hipError_t launch_fast(const rkdg::StageArgs<Real>& A) {
-        return rkdg::launch_stage<Real, true>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block);
+        return rkdg::launch_stage<Real, true>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block,
+                                              persistent_grid);
     }
 
     Engine(rkdg::Model& m, hipStream_t s, bool fast_form) : M(m), stream(s), fast(fast_form) {
@@ -119,12 +136,15 @@ struct Engine final : EngineBase {
         const int ns = M.stepper.nstages;
         q.assign(ns + 1, nullptr);
         sol.assign(ns, nullptr);
+        // every slot starts zeroed: ensemble padding elements are never written, and the limiter's
+        // NaN scan reads them
         for (int s = 0; s <= ns; ++s)
-            if (!alloc(q[s], S)) return RKDG_ERR_HIP;
+            if (!alloc(q[s], S) || hipMemset(q[s], 0, S * sizeof(Real)) != hipSuccess) return RKDG_ERR_HIP;
         for (int s = 0; s < ns; ++s) {
             bool needed = false;  // a later stage combines this stage's M^-1 rhs
             for (int i = s + 1; i < ns; ++i) needed |= (M.stepper.brk[i][s] != 0.0);
-            if (needed && !alloc(sol[s], S)) return RKDG_ERR_HIP;
+            if (needed && (!alloc(sol[s], S) || hipMemset(sol[s], 0, S * sizeof(Real)) != hipSuccess))
+                return RKDG_ERR_HIP;
         }
         if (!upload(bath, M.bath) || !upload(geom, M.geom) || !upload(egeom, M.egeom) || !upload(vxy, M.vxy) ||
             !upload(gmn2, M.gmn2) || !upload(famp, M.famp) || !upload(fphase, M.fphase) ||
@@ -143,6 +163,7 @@ struct Engine final : EngineBase {
                 hipMemset(troubled, 0, ne) != hipSuccess || hipMemset(limited, 0, 3 * ne) != hipSuccess)
                 return fail("hipMemset", hipGetLastError()), RKDG_ERR_HIP;
         }
+        if (M.ens.n > 0 && !init_ensemble()) return RKDG_ERR_HIP;
         if (!alloc(d_nan, 1)) return RKDG_ERR_HIP;
         if (hipMemcpy(d_nan, &kNoNan, sizeof kNoNan, hipMemcpyHostToDevice) != hipSuccess)
             return fail("hipMemcpy", hipGetLastError()), RKDG_ERR_HIP;
@@ -163,14 +184,49 @@ struct Engine final : EngineBase {
         return e;
     }
 
+    bool init_ensemble() {
+        const auto& E = M.ens;
+        const int n = E.n, ns = M.stepper.nstages;
+        std::vector<int> end(n), zero_i(n, 0);
+        std::vector<double> zero_d(n, 0.0), tr(2 * (size_t)n);
+        for (int i = 0; i < n; ++i) {
+            end[i] = E.base[i] + E.ne[i];
+            const bool no_ramp = std::abs(E.ramp_duration[i]) < std::numeric_limits<double>::epsilon() * 100;
+            tr[2 * i] = 0.0 + E.dt[i] * M.stepper.drk[0];
+            tr[2 * i + 1] = no_ramp ? 1.0 : 0.0;
+        }
+        // per stage and instance: cur_c, then term_c of the terms stage() lists, all dt_i * brk
+        std::vector<double> coef((size_t)ns * n * 9, 0.0);
+        for (int st = 0; st < ns; ++st)
+            for (int i = 0; i < n; ++i) {
+                double* c = &coef[((size_t)st * n + i) * 9];
+                c[0] = E.dt[i] * M.stepper.brk[st][st];
+                int j = 0;
+                for (int k = 0; k < st; ++k) {
+                    if (M.stepper.ark[st][k] == 0.0 && M.stepper.brk[st][k] == 0.0) continue;
+                    c[1 + j++] = E.dt[i] * M.stepper.brk[st][k];
+                }
+            }
+        if (!upload(tile_inst_d, E.tile_inst) || !upload(inst_end_d, end) || !upload(inst_dt_d, E.dt) ||
+            !upload(inst_rd_d, E.ramp_duration) || !upload(drk_d, M.stepper.drk) || !upload(inst_t_d, zero_d) ||
+            !upload(inst_step_d, zero_i) || !upload(inst_stage_d, zero_i) || !upload(inst_tr_d, tr) ||
+            !upload(coef_d, coef) || !alloc(step_ctr_d, 1))
+            return false;
+        return hipMemset(step_ctr_d, 0, sizeof(unsigned long long)) == hipSuccess ||
+               fail("hipMemset", hipGetLastError());
+    }
+
     rkdg_status stage() override {
         rkdg::Stepper& st = M.stepper;
         const int s = st.stage, ns = st.nstages;
         rkdg::StageArgs<Real> A;
         std::memset(&A, 0, sizeof A);
+        const bool ens = M.ens.n > 0;
         A.ne = M.ne, A.nedge = M.nedge;
         A.q_s = q[s];
-        A.q_out = q[s + 1];
+        // an ensemble writes its last stage in place into q0 (each element reads only its own q0),
+        // so a captured step replays with fixed pointers
+        A.q_out = (ens && s + 1 == ns) ? q[0] : q[s + 1];
         A.sol_out = sol[s];
         for (int k = 0; k < s; ++k) {
             const double a = st.ark[s][k], c = st.dt * st.brk[s][k];
@@ -202,18 +258,32 @@ struct Engine final : EngineBase {
         A.nan_check = M.prm.slope_limiting ? 0 : 1;
         A.nan_key = d_nan;
         A.stage_key = (unsigned long long)st.step * ns + s;
+        if (ens) {
+            A.tile_inst = tile_inst_d, A.inst_end = inst_end_d, A.inst_tr = inst_tr_d;
+            A.inst_coef = coef_d + (size_t)s * M.ens.n * 9;
+            A.step_ctr = step_ctr_d, A.key_ns = ns, A.stage_key = (unsigned long long)s;
+        }
 
+        const bool tm = timing && !capturing;
         hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (timing) e0 = take_event(), hipEventRecord(e0, stream);
+        if (tm) e0 = take_event(), hipEventRecord(e0, stream);
         hipError_t e = (fast ? launch_fast(A) : launch_plain(A));
         if (e != hipSuccess) return fail("stage kernel launch", e), RKDG_ERR_HIP;
-        if (timing) e1 = take_event(), hipEventRecord(e1, stream), stage_ev.push_back({e0, e1});
+        if (tm) e1 = take_event(), hipEventRecord(e1, stream), stage_ev.push_back({e0, e1});
         ++stages_timed;
 
-        if (s + 1 == ns) std::swap(q[0], q[ns]);  // UpdateState's final swap
+        if (!ens && s + 1 == ns) std::swap(q[0], q[ns]);  // UpdateState's final swap
         const unsigned long long key = A.stage_key;
         st.advance();
-        if (M.prm.wetting_drying) wcur ^= 1;
+        if (M.prm.wetting_drying) {
+            if (ens && s + 1 == ns && (ns & 1)) {
+                // keep the step-start buffer fixed for graph replay: copy the new flags back
+                e = rkdg::launch_copy_bytes((size_t)M.ne, wet[wcur ^ 1], wet[wcur], stream);
+                if (e != hipSuccess) return fail("wet copy launch", e), RKDG_ERR_HIP;
+            } else {
+                wcur ^= 1;
+            }
+        }
 
         if (M.prm.slope_limiting) {
             rkdg::LimiterArgs<Real> L;
@@ -227,12 +297,97 @@ struct Engine final : EngineBase {
             L.baryctr = baryctr, L.set_a = set_a, L.set_b = set_b, L.troubled = troubled, L.limited = limited;
             L.g = M.prm.g, L.M = M.sl_M, L.nu = M.sl_nu, L.p = M.p;
             L.nan_check = 1, L.nan_key = d_nan, L.stage_key = key;
+            if (ens) L.step_ctr = step_ctr_d, L.key_ns = ns;
             hipEvent_t l0 = nullptr, l1 = nullptr;
-            if (timing) l0 = take_event(), hipEventRecord(l0, stream);
+            if (tm) l0 = take_event(), hipEventRecord(l0, stream);
             e = rkdg::launch_limiter<Real>(M.p, L, stream, block);
             if (e != hipSuccess) return fail("limiter kernel launch", e), RKDG_ERR_HIP;
-            if (timing) l1 = take_event(), hipEventRecord(l1, stream), lim_ev.push_back({l0, l1});
+            if (tm) l1 = take_event(), hipEventRecord(l1, stream), lim_ev.push_back({l0, l1});
         }
+        if (ens) {  // per-instance stepper advance, after the NaN keys of this stage are taken
+            e = rkdg::launch_batch_advance(M.ens.n, ns, inst_t_d, inst_dt_d, inst_rd_d, inst_step_d, inst_stage_d, drk_d,
+                                           inst_tr_d, step_ctr_d, stream);
+            if (e != hipSuccess) return fail("advance launch", e), RKDG_ERR_HIP;
+        }
+        return RKDG_OK;
+    }
+
+    // ---- ensemble instance access (at a step boundary: the state is in q0)
+    rkdg_status inst_copy(int i, double* qh, const double* qin, uint8_t* wh, const uint8_t* win) {
+        if (M.ens.n == 0 || i < 0 || i >= M.ens.n) return fail("bad instance", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (M.stepper.stage != 0) return fail("instance access needs a step boundary", hipErrorInvalidValue), RKDG_ERR_ARG;
+        const size_t base = M.ens.base[i], ne = M.ens.ne[i], rows = 3 * (size_t)M.ndof;
+        hipError_t e = hipStreamSynchronize(stream);
+        std::vector<Real> tmp(rows * ne);
+        if (e == hipSuccess && (qh || qin)) {
+            if (qin) {
+                for (size_t k = 0; k < tmp.size(); ++k) tmp[k] = (Real)qin[k];
+                e = hipMemcpy2D(q[0] + base, M.ne * sizeof(Real), tmp.data(), ne * sizeof(Real), ne * sizeof(Real), rows,
+                                hipMemcpyHostToDevice);
+            } else {
+                e = hipMemcpy2D(tmp.data(), ne * sizeof(Real), q[0] + base, M.ne * sizeof(Real), ne * sizeof(Real), rows,
+                                hipMemcpyDeviceToHost);
+                for (size_t k = 0; k < tmp.size() && e == hipSuccess; ++k) qh[k] = (double)tmp[k];
+            }
+        }
+        if (e == hipSuccess && win) e = hipMemcpy(wet[wcur] + base, win, ne, hipMemcpyHostToDevice);
+        if (e == hipSuccess && wh) e = hipMemcpy(wh, wet[wcur] + base, ne, hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail("instance copy", e), RKDG_ERR_HIP;
+        return RKDG_OK;
+    }
+    rkdg_status inst_get(int i, double* qh, uint8_t* wh, uint8_t* lh, uint8_t* th) override {
+        rkdg_status st = inst_copy(i, qh, nullptr, wh, nullptr);
+        if (st != RKDG_OK) return st;
+        const size_t base = M.ens.base[i], ne = M.ens.ne[i];
+        hipError_t e = hipSuccess;
+        if (lh) {
+            if (limited)
+                e = hipMemcpy2D(lh, ne, limited + base, M.ne, ne, 3, hipMemcpyDeviceToHost);
+            else
+                std::memset(lh, 0, 3 * ne);
+        }
+        if (th && e == hipSuccess) {
+            if (troubled)
+                e = hipMemcpy(th, troubled + base, ne, hipMemcpyDeviceToHost);
+            else
+                std::memset(th, 0, ne);
+        }
+        if (e != hipSuccess) return fail("instance flags", e), RKDG_ERR_HIP;
+        return RKDG_OK;
+    }
+    rkdg_status inst_set(int i, const double* qh, const uint8_t* wh) override {
+        return inst_copy(i, nullptr, qh, nullptr, wh);
+    }
+    rkdg_status inst_time(int i, double* t, int* step, int* stage) override {
+        if (M.ens.n == 0 || i < 0 || i >= M.ens.n) return RKDG_ERR_ARG;
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess && t) e = hipMemcpy(t, inst_t_d + i, sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && step) e = hipMemcpy(step, inst_step_d + i, sizeof(int), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && stage) e = hipMemcpy(stage, inst_stage_d + i, sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail("instance time", e), RKDG_ERR_HIP;
+        return RKDG_OK;
+    }
+    rkdg_status snapshot(bool restore) override {
+        if (M.ens.n == 0 || M.stepper.stage != 0) return RKDG_ERR_ARG;
+        const size_t n = M.ens.n, S = 3 * (size_t)M.ndof * M.ne;
+        struct Buf { void* dev; size_t bytes; };
+        const Buf bufs[] = {{q[0], S * sizeof(Real)}, {wet[0], (size_t)M.ne}, {wet[1], (size_t)M.ne},
+                            {inst_t_d, n * sizeof(double)}, {inst_tr_d, 2 * n * sizeof(double)},
+                            {inst_step_d, n * sizeof(int)}, {inst_stage_d, n * sizeof(int)},
+                            {step_ctr_d, sizeof(unsigned long long)}};
+        const size_t nb = sizeof bufs / sizeof bufs[0];
+        hipError_t e = hipStreamSynchronize(stream);
+        if (!restore && snap.empty())
+            for (size_t k = 0; k < nb && e == hipSuccess; ++k) {
+                void* p = nullptr;
+                e = hipMalloc(&p, bufs[k].bytes);
+                if (e == hipSuccess) snap.push_back(p), allocs.push_back(p);
+            }
+        if (snap.size() != nb) return fail("snapshot", e == hipSuccess ? hipErrorInvalidValue : e), RKDG_ERR_HIP;
+        for (size_t k = 0; k < nb && e == hipSuccess; ++k)
+            e = restore ? hipMemcpy(bufs[k].dev, snap[k], bufs[k].bytes, hipMemcpyDeviceToDevice)
+                        : hipMemcpy(snap[k], bufs[k].dev, bufs[k].bytes, hipMemcpyDeviceToDevice);
+        if (e != hipSuccess) return fail("snapshot copy", e), RKDG_ERR_HIP;
         return RKDG_OK;
     }
 
@@ -322,7 +477,40 @@ struct rkdg_ctx {
     rkdg_perf_report acc{};
     bool timing = false;
     double total_ms = 0;
+    // ensemble: one merged model and engine per polynomial order
+    std::vector<std::unique_ptr<rkdg::Model>> gm;
+    std::vector<std::unique_ptr<EngineBase>> ge;
+    std::vector<std::array<int, 2>> loc;  // instance -> (group, index in group)
+    rkdg_schedule sched{128, 0, 0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    bool batch() const { return !ge.empty(); }
+    int nstages() const { return batch() ? gm[0]->stepper.nstages : model.stepper.nstages; }
+    void drop_graph() {
+        if (gexec) hipGraphExecDestroy(gexec), gexec = nullptr;
+        if (graph) hipGraphDestroy(graph), graph = nullptr;
+    }
 };
+
+// Algorithmic HBM bytes of one stage launch averaged over the scheme's stages: every array element
+// the kernel must read or write, each counted once; ne_real excludes ensemble padding.
+static double stage_bytes(const rkdg::Model& M, rkdg_precision precision, double ne_real) {
+    const double b = precision == RKDG_FP32_BULK ? 4.0 : 8.0;
+    const double S = 3.0 * M.ndof * b;
+    double reads_q = 0;
+    const int ns = M.stepper.nstages;
+    for (int s = 0; s < ns; ++s) {
+        reads_q += 1;  // q_s
+        for (int k = 0; k < s; ++k) {
+            if (M.stepper.ark[s][k] != 0.0 || M.stepper.brk[s][k] != 0.0) reads_q += 1;
+            if (M.stepper.brk[s][k] != 0.0) reads_q += 1;
+        }
+    }
+    reads_q /= ns;
+    double per_elt = (reads_q + 1.0) * S + M.ndof * b + 5 * b + 28.0;
+    if (M.prm.wetting_drying) per_elt += 2.0 + 24.0;
+    return per_elt * ne_real + 3.0 * b * M.nedge;
+}
 
 static rkdg_status set_err(rkdg_ctx* c, rkdg_status s, const std::string& msg) {
     if (c) c->err = msg;
@@ -385,6 +573,8 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
 void rkdg_destroy(rkdg_ctx* ctx) {
     if (!ctx) return;
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    ctx->drop_graph();
+    ctx->ge.clear();
     ctx->eng.reset();
     if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -394,26 +584,91 @@ const char* rkdg_last_error(const rkdg_ctx* ctx) { return ctx ? ctx->err.c_str()
 
 rkdg_status rkdg_set_state(rkdg_ctx* ctx, const double* q, const uint8_t* wet) {
     if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     rkdg_status s = ctx->eng->set_state(q, wet);
     return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
 }
 
 rkdg_status rkdg_get_state(rkdg_ctx* ctx, double* q, uint8_t* wet, uint8_t* limited, uint8_t* troubled) {
     if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     rkdg_status s = ctx->eng->get_state(q, wet, limited, troubled);
     return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
 }
 
 static rkdg_status check_nan(rkdg_ctx* ctx) {
-    unsigned long long key = kNoNan;
-    rkdg_status s = ctx->eng->nan_key(&key);
-    if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
-    if (key != kNoNan) return set_err(ctx, RKDG_ERR_NAN, "NaN found in the state");
+    std::vector<EngineBase*> engines;
+    if (ctx->batch())
+        for (auto& g : ctx->ge) engines.push_back(g.get());
+    else
+        engines.push_back(ctx->eng.get());
+    for (EngineBase* g : engines) {
+        unsigned long long key = kNoNan;
+        rkdg_status s = g->nan_key(&key);
+        if (s != RKDG_OK) return set_err(ctx, s, g->err);
+        if (key != kNoNan) return set_err(ctx, RKDG_ERR_NAN, "NaN found in the state");
+    }
+    return RKDG_OK;
+}
+
+// One ensemble step: every stage of every order group, eagerly or by replaying the captured graph.
+static rkdg_status batch_capture(rkdg_ctx* ctx) {
+    ctx->drop_graph();
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return set_err(ctx, RKDG_ERR_HIP, std::string("hipStreamBeginCapture: ") + hipGetErrorString(e));
+    rkdg_status st = RKDG_OK;
+    const int ns = ctx->nstages();
+    for (int k = 0; k < ns && st == RKDG_OK; ++k)
+        for (auto& g : ctx->ge) {
+            g->capturing = true;
+            st = g->stage();
+            g->capturing = false;
+            if (st != RKDG_OK) {
+                set_err(ctx, st, g->err);
+                break;
+            }
+        }
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != RKDG_OK) {
+        if (graph) hipGraphDestroy(graph);
+        return st;
+    }
+    if (e != hipSuccess) return set_err(ctx, RKDG_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    ctx->graph = graph;
+    e = hipGraphInstantiate(&ctx->gexec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        ctx->drop_graph();
+        return set_err(ctx, RKDG_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+    return RKDG_OK;
+}
+
+static rkdg_status batch_steps(rkdg_ctx* ctx, int nsteps) {
+    if (ctx->sched.graph) {
+        if (!ctx->gexec) {
+            rkdg_status st = batch_capture(ctx);
+            if (st != RKDG_OK) return st;
+        }
+        for (int i = 0; i < nsteps; ++i) {
+            hipError_t e = hipGraphLaunch(ctx->gexec, ctx->stream);
+            if (e != hipSuccess) return set_err(ctx, RKDG_ERR_HIP, std::string("hipGraphLaunch: ") + hipGetErrorString(e));
+        }
+        return RKDG_OK;
+    }
+    const int ns = ctx->nstages();
+    for (int i = 0; i < nsteps; ++i)
+        for (int k = 0; k < ns; ++k)
+            for (auto& g : ctx->ge) {
+                rkdg_status st = g->stage();
+                if (st != RKDG_OK) return set_err(ctx, st, g->err);
+            }
     return RKDG_OK;
 }
 
 rkdg_status rkdg_stage(rkdg_ctx* ctx) {
     if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     rkdg_status s = ctx->eng->stage();
     if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
     return check_nan(ctx);
@@ -426,12 +681,17 @@ rkdg_status rkdg_step(rkdg_ctx* ctx, int32_t nsteps) {
         hipEventCreate(&t0), hipEventCreate(&t1);
         hipEventRecord(t0, ctx->stream);
     }
-    const int ns = ctx->model.stepper.nstages;
-    for (int i = 0; i < nsteps; ++i)
-        for (int k = 0; k < ns; ++k) {
-            rkdg_status s = ctx->eng->stage();
-            if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
-        }
+    if (ctx->batch()) {
+        rkdg_status s = batch_steps(ctx, nsteps);
+        if (s != RKDG_OK) return s;
+    } else {
+        const int ns = ctx->model.stepper.nstages;
+        for (int i = 0; i < nsteps; ++i)
+            for (int k = 0; k < ns; ++k) {
+                rkdg_status s = ctx->eng->stage();
+                if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
+            }
+    }
     if (ctx->timing) {
         hipEventRecord(t1, ctx->stream);
         hipEventSynchronize(t1);
@@ -451,6 +711,7 @@ rkdg_status rkdg_sync(rkdg_ctx* ctx) {
 
 rkdg_status rkdg_get_time(rkdg_ctx* ctx, double* t, int32_t* step, int32_t* stage) {
     if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     if (t) *t = ctx->model.stepper.t;
     if (step) *step = ctx->model.stepper.step;
     if (stage) *stage = ctx->model.stepper.stage;
@@ -459,6 +720,7 @@ rkdg_status rkdg_get_time(rkdg_ctx* ctx, double* t, int32_t* step, int32_t* stag
 
 rkdg_status rkdg_set_time(rkdg_ctx* ctx, double t, int32_t step) {
     if (!ctx || step < 0) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     rkdg::Stepper& st = ctx->model.stepper;
     if (st.stage != 0) return set_err(ctx, RKDG_ERR_ARG, "rkdg_set_time needs a step boundary (stage 0)");
     st.t = t;
@@ -470,6 +732,7 @@ rkdg_status rkdg_set_time(rkdg_ctx* ctx, double t, int32_t step) {
 
 rkdg_status rkdg_nan_info(rkdg_ctx* ctx, int32_t* element, int32_t* step, int32_t* stage) {
     if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     unsigned long long key = kNoNan;
     rkdg_status s = ctx->eng->nan_key(&key);
     if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
@@ -489,6 +752,7 @@ rkdg_status rkdg_nan_info(rkdg_ctx* ctx, int32_t* element, int32_t* step, int32_
 
 rkdg_status rkdg_l2_residual(rkdg_ctx* ctx, double* l2) {
     if (!ctx || !l2) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     const rkdg::Model& M = ctx->model;
     std::vector<double> q(3 * (size_t)M.ndof * M.ne);
     rkdg_status s = ctx->eng->get_state(q.data(), nullptr, nullptr, nullptr);
@@ -501,9 +765,16 @@ rkdg_status rkdg_reset_timers(rkdg_ctx* ctx) {
     if (!ctx) return RKDG_ERR_ARG;
     hipStreamSynchronize(ctx->stream);
     rkdg_perf_report scratch{};
-    ctx->eng->collect(&scratch);
-    ctx->eng->timing = true;
-    ctx->eng->stages_timed = 0;
+    std::vector<EngineBase*> engines;
+    if (ctx->batch())
+        for (auto& g : ctx->ge) engines.push_back(g.get());
+    else
+        engines.push_back(ctx->eng.get());
+    for (EngineBase* g : engines) {
+        g->collect(&scratch);
+        g->timing = true;
+        g->stages_timed = 0;
+    }
     ctx->acc = rkdg_perf_report{};
     ctx->timing = true;
     ctx->total_ms = 0;
@@ -512,6 +783,29 @@ rkdg_status rkdg_reset_timers(rkdg_ctx* ctx) {
 
 rkdg_status rkdg_report(rkdg_ctx* ctx, rkdg_perf_report* r) {
     if (!ctx || !r) return RKDG_ERR_ARG;
+    if (ctx->batch()) {
+        int64_t staged = 0, ne = 0;
+        double bytes = 0;
+        for (size_t k = 0; k < ctx->ge.size(); ++k) {
+            ctx->ge[k]->collect(&ctx->acc);
+            staged = std::max(staged, ctx->ge[k]->stages_timed);
+            const rkdg::Model& M = *ctx->gm[k];
+            int64_t real = 0;
+            for (int n : M.ens.ne) real += n;
+            ne += real;
+            bytes += stage_bytes(M, ctx->precision, (double)real);
+        }
+        *r = ctx->acc;
+        r->n_elements = ne;
+        r->p = ctx->gm.size() == 1 ? ctx->gm[0]->p : 0;
+        r->ndof = ctx->gm.size() == 1 ? ctx->gm[0]->ndof : 0;
+        r->nstages = ctx->nstages();
+        r->precision = ctx->precision;
+        r->stages_timed = staged;
+        r->total_ms = ctx->total_ms;
+        r->stage_bytes_alg = bytes;  // one stage of every group
+        return RKDG_OK;
+    }
     const rkdg::Model& M = ctx->model;
     ctx->eng->collect(&ctx->acc);
     const int64_t staged = ctx->eng->stages_timed;
@@ -521,23 +815,234 @@ rkdg_status rkdg_report(rkdg_ctx* ctx, rkdg_perf_report* r) {
     r->precision = ctx->precision;
     r->stages_timed = staged;
     r->total_ms = ctx->total_ms;
-    // Algorithmic HBM bytes of one stage launch averaged over the scheme's stages: every
-    // array element the kernel must read or write, each counted once.
-    const double b = ctx->precision == RKDG_FP32_BULK ? 4.0 : 8.0;
-    const double S = 3.0 * M.ndof * b;
-    double reads_q = 0;
-    const int ns = M.stepper.nstages;
-    for (int s = 0; s < ns; ++s) {
-        reads_q += 1;  // q_s
-        for (int k = 0; k < s; ++k) {
-            if (M.stepper.ark[s][k] != 0.0 || M.stepper.brk[s][k] != 0.0) reads_q += 1;
-            if (M.stepper.brk[s][k] != 0.0) reads_q += 1;
-        }
+    r->stage_bytes_alg = stage_bytes(M, ctx->precision, (double)M.ne);
+    return RKDG_OK;
+}
+
+// ---- ensembles
+rkdg_status rkdg_create_batch(int32_t n, const rkdg_mesh_desc* meshes, const rkdg_params* params,
+                              rkdg_precision precision, rkdg_ctx** out) {
+    g_create_error.clear();
+    if (n < 1 || !meshes || !params || !out) {
+        g_create_error = "null argument or empty batch";
+        return RKDG_ERR_ARG;
     }
-    reads_q /= ns;
-    double per_elt = (reads_q + 1.0) * S + M.ndof * b + 5 * b + 28.0;
-    if (M.prm.wetting_drying) per_elt += 2.0 + 24.0;
-    r->stage_bytes_alg = per_elt * M.ne + 3.0 * b * M.nedge;
+    *out = nullptr;
+    if (precision != RKDG_FP64_PLAIN && precision != RKDG_FP32_BULK && precision != RKDG_FP64_FAST) {
+        g_create_error = "unknown precision";
+        return RKDG_ERR_ARG;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device visible";
+        return RKDG_ERR_NO_DEVICE;
+    }
+    std::unique_ptr<rkdg_ctx> c(new (std::nothrow) rkdg_ctx());
+    if (!c) return RKDG_ERR_ARG;
+    c->precision = precision;
+    // build each instance, then merge per polynomial order
+    std::vector<std::unique_ptr<rkdg::Model>> parts(n);
+    std::map<int, std::vector<int>> by_p;
+    for (int i = 0; i < n; ++i) {
+        parts[i].reset(new rkdg::Model());
+        std::string msg;
+        try {
+            msg = rkdg::build_model(meshes[i], params[i], *parts[i], precision == RKDG_FP64_PLAIN);
+        } catch (const std::exception& ex) {
+            msg = ex.what();
+        }
+        if (msg.empty() && !(params[i].dt > 0)) msg = "dt must be positive";
+        if (!msg.empty()) {
+            g_create_error = "instance " + std::to_string(i) + ": " + msg;
+            return RKDG_ERR_ARG;
+        }
+        by_p[params[i].p].push_back(i);
+    }
+    c->loc.assign(n, {0, 0});
+    for (auto& kv : by_p) {
+        std::vector<const rkdg::Model*> ms;
+        for (int i : kv.second) {
+            c->loc[i] = {(int)c->gm.size(), (int)ms.size()};
+            ms.push_back(parts[i].get());
+        }
+        std::unique_ptr<rkdg::Model> g(new rkdg::Model());
+        std::string msg = rkdg::merge_models(ms, *g);
+        if (!msg.empty()) {
+            g_create_error = msg;
+            return RKDG_ERR_ARG;
+        }
+        for (int i : kv.second) parts[i].reset();  // free the instance copies as groups complete
+        c->gm.push_back(std::move(g));
+    }
+    for (size_t k = 1; k < c->gm.size(); ++k)
+        if (c->gm[k]->stepper.nstages != c->gm[0]->stepper.nstages) {
+            g_create_error = "all instances need the same number of Runge-Kutta stages";
+            return RKDG_ERR_ARG;
+        }
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        g_create_error = "hipStreamCreate failed";
+        return RKDG_ERR_HIP;
+    }
+    for (auto& g : c->gm) {
+        std::unique_ptr<EngineBase> eng;
+        if (precision == RKDG_FP32_BULK)
+            eng.reset(new Engine<float>(*g, c->stream, true));
+        else
+            eng.reset(new Engine<double>(*g, c->stream, precision == RKDG_FP64_FAST));
+        rkdg_status s = eng->init();
+        if (s != RKDG_OK) {
+            g_create_error = eng->err;
+            c->ge.clear();
+            hipStreamDestroy(c->stream);
+            return s;
+        }
+        eng->block = c->sched.block;
+        c->ge.push_back(std::move(eng));
+    }
+    *out = c.release();
+    return RKDG_OK;
+}
+
+int32_t rkdg_batch_size(const rkdg_ctx* ctx) { return ctx ? (int32_t)ctx->loc.size() : 0; }
+
+static EngineBase* batch_engine(rkdg_ctx* ctx, int32_t i, int* local) {
+    if (!ctx || !ctx->batch() || i < 0 || i >= (int32_t)ctx->loc.size()) return nullptr;
+    *local = ctx->loc[i][1];
+    return ctx->ge[ctx->loc[i][0]].get();
+}
+
+rkdg_status rkdg_batch_get_state(rkdg_ctx* ctx, int32_t instance, double* q, uint8_t* wet, uint8_t* limited,
+                                 uint8_t* troubled) {
+    int li = 0;
+    EngineBase* g = batch_engine(ctx, instance, &li);
+    if (!g) return ctx ? set_err(ctx, RKDG_ERR_ARG, "not a batch context or bad instance") : RKDG_ERR_ARG;
+    rkdg_status s = g->inst_get(li, q, wet, limited, troubled);
+    return s == RKDG_OK ? s : set_err(ctx, s, g->err);
+}
+
+rkdg_status rkdg_batch_set_state(rkdg_ctx* ctx, int32_t instance, const double* q, const uint8_t* wet) {
+    int li = 0;
+    EngineBase* g = batch_engine(ctx, instance, &li);
+    if (!g) return ctx ? set_err(ctx, RKDG_ERR_ARG, "not a batch context or bad instance") : RKDG_ERR_ARG;
+    rkdg_status s = g->inst_set(li, q, wet);
+    return s == RKDG_OK ? s : set_err(ctx, s, g->err);
+}
+
+rkdg_status rkdg_batch_get_time(rkdg_ctx* ctx, int32_t instance, double* t, int32_t* step, int32_t* stage) {
+    int li = 0;
+    EngineBase* g = batch_engine(ctx, instance, &li);
+    if (!g) return ctx ? set_err(ctx, RKDG_ERR_ARG, "not a batch context or bad instance") : RKDG_ERR_ARG;
+    int st = 0, sg = 0;
+    rkdg_status s = g->inst_time(li, t, &st, &sg);
+    if (step) *step = st;
+    if (stage) *stage = sg;
+    return s == RKDG_OK ? s : set_err(ctx, s, g->err);
+}
+
+rkdg_status rkdg_batch_nan_info(rkdg_ctx* ctx, int32_t* instance, int32_t* element, int32_t* step, int32_t* stage) {
+    if (!ctx || !ctx->batch()) return RKDG_ERR_ARG;
+    if (instance) *instance = -1;
+    if (element) *element = -1;
+    if (step) *step = -1;
+    if (stage) *stage = -1;
+    const int ns = ctx->nstages();
+    unsigned long long best = kNoNan;
+    int best_g = -1;
+    for (size_t k = 0; k < ctx->ge.size(); ++k) {
+        unsigned long long key = kNoNan;
+        rkdg_status s = ctx->ge[k]->nan_key(&key);
+        if (s != RKDG_OK) return set_err(ctx, s, ctx->ge[k]->err);
+        if (key < best) best = key, best_g = (int)k;
+    }
+    if (best_g < 0) return RKDG_OK;
+    const rkdg::Model& M = *ctx->gm[best_g];
+    const int e = (int)(best & 0xffffffffu);
+    for (int li = 0; li < M.ens.n; ++li)
+        if (e >= M.ens.base[li] && e < M.ens.base[li] + M.ens.ne[li]) {
+            for (size_t i = 0; i < ctx->loc.size(); ++i)
+                if (ctx->loc[i][0] == best_g && ctx->loc[i][1] == li && instance) *instance = (int32_t)i;
+            if (element) *element = e - M.ens.base[li];
+        }
+    if (step) *step = (int32_t)((best >> 32) / ns);
+    if (stage) *stage = (int32_t)((best >> 32) % ns);
+    return RKDG_OK;
+}
+
+static bool schedule_valid(const rkdg_schedule& s) {
+    return (s.block == 64 || s.block == 128 || s.block == 256) && s.groups_per_xcd >= 0 &&
+           s.groups_per_xcd <= 4096 && (s.graph == 0 || s.graph == 1);
+}
+
+rkdg_status rkdg_batch_set_schedule(rkdg_ctx* ctx, const rkdg_schedule* sc) {
+    if (!ctx || !sc) return RKDG_ERR_ARG;
+    if (!ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "not a batch context");
+    if (!schedule_valid(*sc)) return set_err(ctx, RKDG_ERR_ARG, "invalid schedule");
+    for (auto& g : ctx->gm)
+        if (sc->groups_per_xcd > 0 && g->closed_forms)
+            return set_err(ctx, RKDG_ERR_ARG, "the persistent kernel is not built for closed-form problems");
+    hipStreamSynchronize(ctx->stream);
+    ctx->sched = *sc;
+    for (auto& g : ctx->ge) g->block = sc->block, g->persistent_grid = 8 * sc->groups_per_xcd;
+    ctx->drop_graph();
+    return RKDG_OK;
+}
+
+rkdg_status rkdg_batch_get_schedule(const rkdg_ctx* ctx, rkdg_schedule* sc) {
+    if (!ctx || !sc || !ctx->batch()) return RKDG_ERR_ARG;
+    *sc = ctx->sched;
+    return RKDG_OK;
+}
+
+rkdg_status rkdg_batch_autotune(rkdg_ctx* ctx, int32_t steps_per_trial, rkdg_schedule* best, double* best_ms) {
+    if (!ctx || steps_per_trial < 1) return RKDG_ERR_ARG;
+    if (!ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "not a batch context");
+    for (auto& g : ctx->ge) {
+        rkdg_status s = g->snapshot(false);
+        if (s != RKDG_OK) return set_err(ctx, s, "autotune needs a step boundary");
+    }
+    bool cf = false;
+    for (auto& g : ctx->gm) cf |= g->closed_forms;
+    int cu = 0;
+    hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, 0);
+    const int cu_per_xcd = std::max(1, cu / 8);
+    std::vector<rkdg_schedule> cand;
+    for (int b : {64, 128, 256})
+        for (int mult : {0, 1, 2, 4})
+            for (int gr : {0, 1}) {
+                if (mult && cf) continue;
+                cand.push_back(rkdg_schedule{b, mult * cu_per_xcd, gr});
+            }
+    const bool was_timing = ctx->timing;
+    rkdg_schedule win = ctx->sched;
+    double win_ms = 1e300;
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipEventCreate(&t0), hipEventCreate(&t1);
+    rkdg_status st = RKDG_OK;
+    for (const rkdg_schedule& sc : cand) {
+        if ((st = rkdg_batch_set_schedule(ctx, &sc)) != RKDG_OK) break;
+        for (auto& g : ctx->ge)
+            if ((st = g->snapshot(true)) != RKDG_OK) break;
+        if (st != RKDG_OK) break;
+        ctx->timing = false;
+        if ((st = batch_steps(ctx, 1)) != RKDG_OK) break;  // warm-up (and graph capture)
+        hipEventRecord(t0, ctx->stream);
+        if ((st = batch_steps(ctx, steps_per_trial)) != RKDG_OK) break;
+        hipEventRecord(t1, ctx->stream);
+        hipEventSynchronize(t1);
+        float ms = 0;
+        hipEventElapsedTime(&ms, t0, t1);
+        const double per = ms / steps_per_trial;
+        if (per < win_ms) win_ms = per, win = sc;
+    }
+    hipEventDestroy(t0), hipEventDestroy(t1);
+    ctx->timing = was_timing;
+    if (st == RKDG_OK) st = check_nan(ctx);
+    for (auto& g : ctx->ge) g->snapshot(true);
+    rkdg_status s2 = rkdg_batch_set_schedule(ctx, &win);
+    if (st != RKDG_OK) return st;
+    if (s2 != RKDG_OK) return s2;
+    if (best) *best = win;
+    if (best_ms) *best_ms = win_ms;
     return RKDG_OK;
 }
 

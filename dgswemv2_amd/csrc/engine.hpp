@@ -83,8 +83,34 @@ struct StageArgs {
     double h_o, h_thr;
     int nan_check;           // scan the new state for NaN here (no limiter after this stage)
     unsigned long long* nan_key;   // min over NaN finds of (stage_key << 32 | element)
-    unsigned long long stage_key;  // step * nstages + stage of this launch
+    unsigned long long stage_key;  // step * nstages + stage of this launch (the stage alone with step_ctr)
+    // Ensembles: several meshes merged into one element range, each padded to whole 256-element
+    // tiles, with per-instance time, ramp and SSP-RK coefficients.  All null for a single mesh.
+    const int* tile_inst;    // [ne/64] instance of each 64-element tile
+    const int* inst_end;     // [ninst] one past the instance's last element
+    const double* inst_tr;   // [ninst][2] stage time and ramp of each instance
+    const Real* inst_coef;   // [ninst][9] cur_c, term_c[0..7] (dt * brk) of this stage per instance
+    const unsigned long long* step_ctr;  // device step counter: stage_key = step * key_ns + stage_key
+    int key_ns;
 };
+
+// Per-element view of the instance-dependent stage scalars (single mesh: the launch's own).
+template <typename Real>
+struct InstScal {
+    double t, ramp;
+    const Real* coef;  // null: use A.cur_c / A.term_c
+};
+
+// Instance of element e in an ensemble (uniform over a work-group's tile), or -1.
+template <typename Real>
+RKDG_HD inline int instance_of(const StageArgs<Real>& A, int e) {
+    return A.tile_inst ? A.tile_inst[e >> 6] : -1;
+}
+template <typename Real>
+RKDG_HD inline InstScal<Real> inst_scal(const StageArgs<Real>& A, int inst) {
+    if (inst < 0) return InstScal<Real>{A.t, A.ramp, nullptr};
+    return InstScal<Real>{A.inst_tr[2 * inst], A.inst_tr[2 * inst + 1], A.inst_coef + 9 * (size_t)inst};
+}
 
 // ---------------------------------------------------------------------------------------
 // Element-fastest arrays: entry i of column c sits at base[c * n + i].  On the GPU a column
@@ -399,14 +425,14 @@ struct FaceTables {
 
 // Tide/flow exterior forcing value at one edge point (rkdg_swe_bc_tide.hpp:76-83).
 template <typename Real, int NE>
-RKDG_HD inline Real harmonic_forcing(const StageArgs<Real>& A, int ed, int g) {
+RKDG_HD inline Real harmonic_forcing(const StageArgs<Real>& A, int ed, int g, double t, double ramp) {
     const int fr = A.eforce[ed];
     Real acc = Real(0.0);
     for (int c = 0; c < A.ncon; ++c) {
         using std::cos;
         const Real amp = A.famp[((size_t)fr * A.ncon + c) * NE + g];
         const Real ph = A.fphase[((size_t)fr * A.ncon + c) * NE + g];
-        acc += Real(A.ramp) * A.ffact[c] * amp * cos(A.freq[c] * Real(A.t) + (A.eqarg[c] - ph) * Real(kPI) / Real(180));
+        acc += Real(ramp) * A.ffact[c] * amp * cos(A.freq[c] * Real(t) + (A.eqarg[c] - ph) * Real(kPI) / Real(180));
     }
     return acc;
 }
@@ -448,7 +474,8 @@ RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Re
             has_nan |= (nq[k][d] != nq[k][d]);
         }
     if (A.nan_check && has_nan) {
-        const unsigned long long key = (A.stage_key << 32) | (unsigned long long)(unsigned)e;
+        const unsigned long long sk = A.step_ctr ? *A.step_ctr * (unsigned long long)A.key_ns + A.stage_key : A.stage_key;
+        const unsigned long long key = (sk << 32) | (unsigned long long)(unsigned)e;
 #ifdef __HIP_DEVICE_COMPILE__
         atomicMin(A.nan_key, key);
 #else
@@ -461,7 +488,7 @@ RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Re
 // The fused stage of one element, reference arithmetic order (FP64 plain form).
 
 template <typename Real, int P, bool WD, bool CF>
-RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e) {
+RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e, const InstScal<Real>& IS) {
     using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
@@ -565,7 +592,7 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
                 const Real y = (A.vxy[3 * (size_t)ne + e] * C::psi[0][g] + A.vxy[4 * (size_t)ne + e] * C::psi[1][g]) +
                                A.vxy[5 * (size_t)ne + e] * C::psi[2][g];
                 Real f[3];
-                problem_source<Real>(A.t, x, y, f);
+                problem_source<Real>(IS.t, x, y, f);
 #pragma unroll
                 for (int k = 0; k < 3; ++k) src[k] = src[k] + f[k];
             }
@@ -701,7 +728,7 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
                     const Real qt_ex = qi[1] * tx + qi[2] * ty;
                     qe[0] = qi[0], qe[1] = qn_ex * nx + qt_ex * tx, qe[2] = qn_ex * ny + qt_ex * ty;
                 } else if (bc == 1 || bc == 2) {  // Tide / Flow (rkdg_swe_bc_tide.hpp:70-96, rkdg_swe_bc_flow.hpp:72-101)
-                    const Real acc = harmonic_forcing<Real, NE>(A, ed, g);
+                    const Real acc = harmonic_forcing<Real, NE>(A, ed, g, IS.t, IS.ramp);
                     if (bc == 1)
                         qe[0] = acc, qe[1] = qi[1], qe[2] = qi[2];
                     else
@@ -712,7 +739,7 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
                                    A.vxy[2 * (size_t)ne + e] * ps[2 * NE + g];
                     const Real y = (A.vxy[3 * (size_t)ne + e] * ps[g] + A.vxy[4 * (size_t)ne + e] * ps[NE + g]) +
                                    A.vxy[5 * (size_t)ne + e] * ps[2 * NE + g];
-                    problem_ic<Real>(A.problem, A.pparam, grav, A.t, x, y, qe);
+                    problem_ic<Real>(A.problem, A.pparam, grav, IS.t, x, y, qe);
                 } else {
                     qe[0] = qi[0], qe[1] = qi[1], qe[2] = qi[2];  // unreachable: function BCs launch the CF build
                 }
@@ -747,10 +774,11 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
             Real acc = Real(0.0);
             for (int i = 0; i < A.nterm; ++i) {
                 Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
-                if (A.term_sol[i]) term = term + A.term_c[i] * ldc(A.term_sol[i], ne, k * ND + d, e);
+                if (A.term_sol[i])
+                    term = term + (IS.coef ? IS.coef[1 + i] : A.term_c[i]) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc = acc + term;
             }
-            acc = acc + (A.cur_a * q[k][d] + A.cur_c * sol);
+            acc = acc + (A.cur_a * q[k][d] + (IS.coef ? IS.coef[0] : A.cur_c) * sol);
             nq[k][d] = acc;
         }
     finish_element<Real, P, WD>(A, e, wet, nq);
@@ -896,10 +924,10 @@ RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* q
 // function: rkdg_swe_bc_function.hpp:139-172.
 template <typename Real, int P, bool CF>
 RKDG_HD inline void exterior_state(const StageArgs<Real>& A, int bc, int e, int ed, int f, int g, const Real* qi,
-                                   Real nx, Real ny, Real* qe) {
+                                   Real nx, Real ny, Real* qe, double t, double ramp) {
     constexpr int NE = Sizes<P>::NE;
     if (bc == 1 || bc == 2) {
-        const Real acc = harmonic_forcing<Real, NE>(A, ed, g);
+        const Real acc = harmonic_forcing<Real, NE>(A, ed, g, t, ramp);
         if (bc == 1)
             qe[0] = acc, qe[1] = qi[1], qe[2] = qi[2];
         else
@@ -911,7 +939,7 @@ RKDG_HD inline void exterior_state(const StageArgs<Real>& A, int bc, int e, int 
                        A.vxy[2 * (size_t)ne + e] * C::psie[f * 3 + 2][g];
         const Real y = A.vxy[3 * (size_t)ne + e] * C::psie[f * 3][g] + A.vxy[4 * (size_t)ne + e] * C::psie[f * 3 + 1][g] +
                        A.vxy[5 * (size_t)ne + e] * C::psie[f * 3 + 2][g];
-        problem_ic<Real>(A.problem, A.pparam, A.g, A.t, x, y, qe);
+        problem_ic<Real>(A.problem, A.pparam, A.g, t, x, y, qe);
     } else {
         qe[0] = qi[0], qe[1] = qi[1], qe[2] = qi[2];  // unreachable: function BCs launch the CF build
     }
@@ -924,7 +952,7 @@ RKDG_HD inline Real fmac_tab(Real a, Real c, Real r) { return c != Real(0) ? fma
 
 template <typename Real, int P, bool WD, bool CF>
 RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phie_lds, const Real* phiew_lds,
-                                       int e) {
+                                       int e, const InstScal<Real>& IS) {
     using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
@@ -1019,7 +1047,7 @@ RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phi
                 const Real y = A.vxy[3 * (size_t)ne + e] * C::psi[0][g] + A.vxy[4 * (size_t)ne + e] * C::psi[1][g] +
                                 A.vxy[5 * (size_t)ne + e] * C::psi[2][g];
                 Real fsv[3];
-                problem_source<Real>(A.t, x, y, fsv);
+                problem_source<Real>(IS.t, x, y, fsv);
                 S[0] += fsv[0], S[1] += fsv[1], S[2] += fsv[2];
             }
             if (A.friction) {
@@ -1198,7 +1226,7 @@ RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phi
                     const Real qn_ex = -(qi[1] * fnx + qi[2] * fny), qt_ex = qi[1] * tx + qi[2] * ty;
                     qe[0] = qi[0], qe[1] = qn_ex * fnx + qt_ex * tx, qe[2] = qn_ex * fny + qt_ex * ty;
                 } else {
-                    exterior_state<Real, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe);
+                    exterior_state<Real, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe, IS.t, IS.ramp);
                     if (bc != 2) zpe = qe[0] - zc;  // tide, function: the exterior elevation is given
                 }
                 const Real po = grav * zo * (zc + Real(0.5) * zo + bo);
@@ -1229,10 +1257,10 @@ RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phi
         for (int d = 0; d < ND; ++d) {
             const Real sol = rhs[k][d] * C::minv[0][d];
             if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
-            Real acc = A.cur_a * q[k][d] + A.cur_c * sol;
+            Real acc = A.cur_a * q[k][d] + (IS.coef ? IS.coef[0] : A.cur_c) * sol;
             for (int i = 0; i < A.nterm; ++i) {
                 Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
-                if (A.term_sol[i]) term += A.term_c[i] * ldc(A.term_sol[i], ne, k * ND + d, e);
+                if (A.term_sol[i]) term += (IS.coef ? IS.coef[1 + i] : A.term_c[i]) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc += term;
             }
             nq[k][d] = acc;

@@ -3,5 +3,5 @@
 #include "kernels.inc"
 
 namespace rkdg {
-template hipError_t launch_stage<double, true>(int, bool, bool, const StageArgs<double>&, hipStream_t, int);
+template hipError_t launch_stage<double, true>(int, bool, bool, const StageArgs<double>&, hipStream_t, int, int);
 }  // namespace rkdg

@@ -40,7 +40,9 @@ struct LimiterArgs {
     int p;
     int nan_check;
     unsigned long long* nan_key;
-    unsigned long long stage_key;
+    unsigned long long stage_key;  // step * nstages + stage (the stage alone with step_ctr)
+    const unsigned long long* step_ctr;  // device step counter of an ensemble, or null
+    int key_ns;
 };
 
 template <typename Real, int P>
@@ -244,7 +246,7 @@ RKDG_HD inline void sl_limit_element(const LimiterArgs<Real>& L, const FaceTable
                 has_nan |= (x != x);
             }
         if (has_nan) {
-            const unsigned long long key = (L.stage_key << 32) | (unsigned long long)(unsigned)e;
+            const unsigned long long key = ((L.step_ctr ? *L.step_ctr * (unsigned long long)L.key_ns + L.stage_key : L.stage_key) << 32) | (unsigned long long)(unsigned)e;
 #ifdef __HIP_DEVICE_COMPILE__
             atomicMin(L.nan_key, key);
 #else

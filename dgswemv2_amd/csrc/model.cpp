@@ -363,24 +363,52 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
     M.wet0.assign(ne, 1);
     M.bath_vrtx.assign(3 * (size_t)ne, 0.0);
     M.gmn2.assign(ne, 0.0);
-    std::vector<double> ipf((size_t)ng * nd), minv((size_t)nd * nd);
+    // per element: v (r x 3 nodal or r x ng point values) -> * psi -> * int_phi_fact -> * M^-1,
+    // each product summed from its first term (the order of the reference's Eigen products);
+    // fixed-size arrays (p <= 2: ndof, ngp <= 6)
+    constexpr int kMax = 6;
+    if (nd > kMax || ng > kMax) return "element tables larger than the device-built set";
+    double ipf[kMax * kMax], minv[kMax];
+    auto project = [&](const double* v, int rows, bool nodal, double* out) {  // out[rows][nd]
+        double at_gp[3][kMax], t[3][kMax];
+        for (int r = 0; r < rows; ++r)
+            for (int g = 0; g < ng; ++g) {
+                if (!nodal) {
+                    at_gp[r][g] = v[r * ng + g];
+                    continue;
+                }
+                double s = v[r * 3 + 0] * psi[g];
+                for (int k = 1; k < 3; ++k) s = s + v[r * 3 + k] * psi[(size_t)k * ng + g];
+                at_gp[r][g] = s;
+            }
+        for (int r = 0; r < rows; ++r)
+            for (int d = 0; d < nd; ++d) {
+                double s = at_gp[r][0] * ipf[d];
+                for (int g = 1; g < ng; ++g) s = s + at_gp[r][g] * ipf[g * nd + d];
+                t[r][d] = s;
+            }
+        // times the diagonal M^-1 through a full product: the zero terms add exact zeros
+        for (int r = 0; r < rows; ++r)
+            for (int d = 0; d < nd; ++d) {
+                double s = t[r][0] * (d == 0 ? minv[0] : 0.0);
+                for (int k = 1; k < nd; ++k) s = s + t[r][k] * (k == d ? minv[k] : 0.0);
+                out[r * nd + d] = s;
+            }
+    };
     for (int e = 0; e < ne; ++e) {
         const double adet = M.geom[4 * (size_t)ne + e];
         for (int g = 0; g < ng; ++g)
-            for (int d = 0; d < nd; ++d) ipf[(size_t)g * nd + d] = m.ipf[(size_t)g * nd + d] * adet;
-        std::fill(minv.begin(), minv.end(), 0.0);
-        for (int d = 0; d < nd; ++d) minv[(size_t)d * nd + d] = m.minv[d] / adet;
+            for (int d = 0; d < nd; ++d) ipf[g * nd + d] = m.ipf[(size_t)g * nd + d] * adet;
+        for (int d = 0; d < nd; ++d) minv[d] = m.minv[d] / adet;
         // bathymetry: L2ProjectionNode of nodal depths (element.hpp:317-321)
-        std::vector<double> bz = {tri[e].z[0], tri[e].z[1], tri[e].z[2]};
-        std::vector<double> b = matmul(matmul(matmul(bz, 1, 3, psi, ng), 1, ng, ipf, nd), 1, nd, minv, nd);
+        double b[kMax], q[3 * kMax];
+        project(tri[e].z, 1, true, b);
         for (int d = 0; d < nd; ++d) M.bath[(size_t)d * ne + e] = b[d];
-        std::vector<double> q;
         if (prm.ic_constant) {
-            std::vector<double> un(9);
-            for (int k = 0; k < 3; ++k) un[k] = prm.ze0, un[3 + k] = prm.qx0, un[6 + k] = prm.qy0;
-            q = matmul(matmul(matmul(un, 3, 3, psi, ng), 3, ng, ipf, nd), 3, nd, minv, nd);
+            const double un[9] = {prm.ze0, prm.ze0, prm.ze0, prm.qx0, prm.qx0, prm.qx0, prm.qy0, prm.qy0, prm.qy0};
+            project(un, 3, true, q);
         } else {  // L2ProjectionF of ic_q(0, x) (element.hpp:307-313)
-            std::vector<double> f(3 * (size_t)ng);
+            double f[3 * kMax];
             for (int g = 0; g < ng; ++g) {
                 const double x = tri[e].x[0] * psi[g] + tri[e].x[1] * psi[ng + g] + tri[e].x[2] * psi[2 * ng + g];
                 const double y = tri[e].y[0] * psi[g] + tri[e].y[1] * psi[ng + g] + tri[e].y[2] * psi[2 * ng + g];
@@ -388,10 +416,10 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
                 problem_ic_host(prm, 0.0, x, y, v);
                 f[g] = v[0], f[ng + g] = v[1], f[2 * ng + g] = v[2];
             }
-            q = matmul(matmul(f, 3, ng, ipf, nd), 3, nd, minv, nd);
+            project(f, 3, false, q);
         }
         for (int k = 0; k < 3; ++k)
-            for (int d = 0; d < nd; ++d) M.q0[(size_t)(k * nd + d) * ne + e] = q[(size_t)k * nd + d];
+            for (int d = 0; d < nd; ++d) M.q0[(size_t)(k * nd + d) * ne + e] = q[k * nd + d];
         for (int v = 0; v < 3; ++v) M.bath_vrtx[(size_t)v * ne + e] = tri[e].z[v];
     }
     if (prm.friction == RKDG_FRICTION_MANNING) {  // :166-197
@@ -488,6 +516,119 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
         }
     }
     return std::string();
+}
+
+// ---------------------------------------------------------------------------------------
+namespace {
+
+// Appends columns of an element-fastest [ncol][ne] array with padding to the merged [ncol][ne_tot]
+// array; `map` transforms the values (e.g. element/edge index offsets).
+template <typename T, typename F>
+void append_cols(std::vector<T>& out, const std::vector<T>& in, int ncol, int ne_in, int ne_tot, int base,
+                 T pad, F map) {
+    if (out.empty()) out.assign((size_t)ncol * ne_tot, pad);
+    for (int c = 0; c < ncol; ++c)
+        for (int e = 0; e < ne_in; ++e) out[(size_t)c * ne_tot + base + e] = map(in[(size_t)c * ne_in + e]);
+}
+template <typename T>
+void append_cols(std::vector<T>& out, const std::vector<T>& in, int ncol, int ne_in, int ne_tot, int base, T pad) {
+    append_cols(out, in, ncol, ne_in, ne_tot, base, pad, [](T v) { return v; });
+}
+
+}  // namespace
+
+std::string merge_models(const std::vector<const Model*>& parts, Model& out) {
+    if (parts.empty()) return "empty ensemble";
+    const Model& a = *parts[0];
+    for (const Model* p : parts) {
+        const Model& b = *p;
+        const rkdg_params &x = a.prm, &y = b.prm;
+        if (b.p != a.p || b.stepper.nstages != a.stepper.nstages || b.stepper.order != a.stepper.order)
+            return "ensemble instances of one group need equal p and Runge-Kutta scheme";
+        if (x.g != y.g || x.friction != y.friction || x.friction_cf != y.friction_cf ||
+            x.wetting_drying != y.wetting_drying || b.h_o != a.h_o || x.slope_limiting != y.slope_limiting ||
+            b.sl_M != a.sl_M || b.sl_nu != a.sl_nu || x.problem != y.problem || x.function_source != y.function_source ||
+            b.reference_face_order != a.reference_face_order)
+            return "ensemble instances need equal physics options (g, friction, wetting-drying, limiter, problem)";
+        if (b.ncon != a.ncon || b.freq != a.freq || b.ffact != a.ffact || b.eqarg != a.eqarg)
+            return "ensemble instances need one harmonic constituent list";
+        for (int i = 0; i < 4; ++i)
+            if (x.problem_param[i] != y.problem_param[i]) return "ensemble instances need equal problem parameters";
+    }
+    if (a.stepper.nstages < 2) return "ensembles need a Runge-Kutta scheme of at least 2 stages";
+    out = Model();
+    out.p = a.p, out.ndof = a.ndof, out.ngp = a.ngp, out.ngpe = a.ngpe;
+    out.master = a.master, out.prm = a.prm, out.h_o = a.h_o, out.h_thr = a.h_thr, out.sl_M = a.sl_M, out.sl_nu = a.sl_nu;
+    out.reference_face_order = a.reference_face_order;
+    out.ncon = a.ncon, out.freq = a.freq, out.ffact = a.ffact, out.eqarg = a.eqarg;
+    out.stepper = a.stepper;
+    auto& E = out.ens;
+    E.n = (int)parts.size();
+    int ne_tot = 0;
+    for (const Model* p : parts) {
+        E.base.push_back(ne_tot);
+        E.ne.push_back(p->ne);
+        E.dt.push_back(p->stepper.dt);
+        E.ramp_duration.push_back(p->stepper.ramp_duration);
+        ne_tot += (p->ne + kEnsembleTile - 1) / kEnsembleTile * kEnsembleTile;
+        if ((size_t)ne_tot * 3 * 6 * sizeof(double) >= ((size_t)1 << 32))
+            return "ensemble too large: state arrays exceed 4 GiB of 32-bit buffer offsets";
+        out.closed_forms |= p->closed_forms;
+    }
+    out.ne = ne_tot;
+    E.tile_inst.assign(ne_tot / 64, 0);
+    for (int i = 0; i < E.n; ++i) {
+        const int t0 = E.base[i] / 64, t1 = (i + 1 < E.n ? E.base[i + 1] : ne_tot) / 64;
+        for (int t = t0; t < t1; ++t) E.tile_inst[t] = i;
+    }
+    const int nd = a.ndof;
+    int nedge_tot = 0;
+    for (const Model* p : parts) nedge_tot += p->nedge;
+    out.egeom.assign(3 * (size_t)nedge_tot, 0.0);
+    int edge_base = 0, force_base = 0, levee_base = 0;
+    for (int i = 0; i < E.n; ++i) {
+        const Model& b = *parts[i];
+        const int base = E.base[i], ne = b.ne;
+        append_cols(out.geom, b.geom, 5, ne, ne_tot, base, 0.0);
+        append_cols(out.bath, b.bath, nd, ne, ne_tot, base, 0.0);
+        append_cols(out.vxy, b.vxy, 6, ne, ne_tot, base, 0.0);
+        append_cols(out.bath_vrtx, b.bath_vrtx, 3, ne, ne_tot, base, 0.0);
+        append_cols(out.gmn2, b.gmn2, 1, ne, ne_tot, base, 0.0);
+        append_cols(out.nb, b.nb, 3, ne, ne_tot, base, -1, [base](int v) { return v >= 0 ? v + base : v; });
+        append_cols(out.edge, b.edge, 3, ne, ne_tot, base, 0, [edge_base](int v) { return v + edge_base; });
+        append_cols(out.meta, b.meta, 1, ne, ne_tot, base, 0u);
+        append_cols(out.neighbor, b.neighbor, 3, ne, ne_tot, base, -1, [base](int v) { return v >= 0 ? v + base : v; });
+        if (a.prm.slope_limiting) {
+            append_cols(out.sl_median, b.sl_median, 6, ne, ne_tot, base, 0.0);
+            append_cols(out.sl_alpha, b.sl_alpha, 6, ne, ne_tot, base, 0.0);
+            append_cols(out.sl_rsq, b.sl_rsq, 3, ne, ne_tot, base, 0.0);
+            append_cols(out.sl_len, b.sl_len, 3, ne, ne_tot, base, 0.0);
+            append_cols(out.sl_radius, b.sl_radius, 1, ne, ne_tot, base, 0.0);
+            append_cols(out.sl_bath_bc, b.sl_bath_bc, 1, ne, ne_tot, base, 0.0);
+            append_cols(out.sl_face_slot, b.sl_face_slot, 3, ne, ne_tot, base, 0);
+        }
+        append_cols(out.q0, b.q0, 3 * nd, ne, ne_tot, base, 0.0);
+        append_cols(out.wet0, b.wet0, 1, ne, ne_tot, base, (uint8_t)0);
+        // edge records: interfaces first (their aux record is a levee record), then boundaries
+        // (their aux record is a tide/flow forcing record)
+        const int ned = b.nedge;
+        for (int c = 0; c < 3; ++c)
+            std::copy(b.egeom.begin() + (size_t)c * ned, b.egeom.begin() + (size_t)(c + 1) * ned,
+                      out.egeom.begin() + (size_t)c * nedge_tot + edge_base);
+        for (int k = 0; k < ned; ++k) {
+            const int r = b.eforce[k];
+            out.eforce.push_back(r < 0 ? -1 : r + (k < b.nintf ? levee_base : force_base));
+        }
+        out.famp.insert(out.famp.end(), b.famp.begin(), b.famp.end());
+        out.fphase.insert(out.fphase.end(), b.fphase.begin(), b.fphase.end());
+        out.levee_gp.insert(out.levee_gp.end(), b.levee_gp.begin(), b.levee_gp.end());
+        edge_base += ned;
+        if (b.ncon) force_base += (int)(b.famp.size() / ((size_t)b.ncon * b.ngpe));
+        levee_base += (int)(b.levee_gp.size() / (3 * (size_t)b.ngpe));
+    }
+    out.nedge = edge_base;
+    out.nintf = 0;  // mixed; the merged eforce already carries the right record bases
+    return "";
 }
 
 double l2_residual(const Model& M, const std::vector<double>& q, double t) {

@@ -66,12 +66,27 @@ struct Model {
     std::vector<uint8_t> wet0;      // [ne]
 
     Stepper stepper;
+
+    // Ensemble layout (set by merge_models): instance i owns elements [base[i], base[i] + ne[i]),
+    // padded to a multiple of kEnsembleTile; padding elements are dry, without neighbours.
+    struct Ensemble {
+        int n = 0;
+        std::vector<int> base, ne;              // per instance
+        std::vector<int> tile_inst;             // per 64-element tile
+        std::vector<double> dt, ramp_duration;  // per instance
+    } ens;
 };
+
+constexpr int kEnsembleTile = 256;  // instance padding: any block size of 64, 128, 256 stays instance-pure
 
 // Builds the model; returns an empty string or an error message.
 // reference_face_order: face slots in the reference's rhs accumulation order (the FP64 form);
 // otherwise slots are the local faces 0, 1, 2 (the FP32 form).
 std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M, bool reference_face_order = true);
+
+// Merges models of equal polynomial order, RK scheme and physics options into one element
+// range (an ensemble); returns an empty string or an error message.
+std::string merge_models(const std::vector<const Model*>& parts, Model& out);
 
 // L2 residual against the problem's true solution (element.hpp:547-592), FP64, host.
 double l2_residual(const Model& M, const std::vector<double>& q, double t);

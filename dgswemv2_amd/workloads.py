@@ -18,6 +18,8 @@ same recipe small.
 """
 import math
 
+import numpy as np
+
 from . import (BC_FUNCTION, BC_LAND, BC_TIDE, FRICTION_MANNING, PROBLEM_DAM_BREAK, PROBLEM_MANUFACTURED,
                PROBLEM_NONE, PROBLEM_PARABOLIC_BOWL, make_params, quad_mesh)
 
@@ -69,6 +71,33 @@ def dambreak(scale=1.0, p=1):
     prm = make_params(p=p, nstages=2, order=2, dt=dt, g=9.81, problem=PROBLEM_DAM_BREAK, ic_constant=False, wetting_drying=True, h_o=0.01,
                       slope_limiting=True, problem_param=(2000.0, 0.0, -9.0, 0.0))
     return mesh, prm
+
+
+def ensemble(n=64, scale=1.0, seed=4, ne_min=20000, ne_max=400000, nstages=3, order=3):
+    """Ensemble of n tidal instances of the tidal recipe (1 km quads, jitter 20 %, Fourier depth
+    5-4000 m, M2 tide on one side, Manning 0.025): triangle counts log-uniform in [ne_min, ne_max]
+    (square grids of m = floor(sqrt(Ne / 2)) quads, Ne scaled by `scale`), p in {1, 2} with
+    probability 1/2, tide amplitude uniform in [0.1, 1.5] m, SSP-RK3, each with its own CFL dt.
+    Returns (meshes, params, info)."""
+    rng = np.random.default_rng(seed)
+    meshes, params, info = [], [], []
+    g = 9.81
+    for i in range(n):
+        ne = math.exp(rng.uniform(math.log(ne_min), math.log(ne_max))) * scale
+        p = int(rng.integers(1, 3))
+        amp = float(rng.uniform(0.1, 1.5))
+        m = max(2, int(math.sqrt(ne / 2)))
+        L = 1000.0 * m
+        mesh = quad_mesh(0.0, 0.0, L, L, m, m, bc=(BC_LAND, BC_LAND, BC_LAND, BC_TIDE), bathy="fourier", bmin=5.0,
+                         bmax=4000.0, n_modes=16, jitter=0.2, seed=seed * 1000 + i, frequency=M2_OMEGA,
+                         forcing_fact=1.0, equilib_arg=0.0, amplitude=amp, phase=0.0, manning_n=0.025)
+        dt = 0.1 * mesh.min_edge_length() / (math.sqrt(g * 4000.0) * (2 * p + 1))
+        prm = make_params(p=p, nstages=nstages, order=order, dt=dt, g=g, problem=PROBLEM_NONE, ic_constant=True,
+                          friction=FRICTION_MANNING, friction_cf=0.0)
+        meshes.append(mesh)
+        params.append(prm)
+        info.append({"triangles": mesh.n_elements, "p": p, "amplitude": amp, "dt": dt})
+    return meshes, params, info
 
 
 WORKLOADS = {"manufactured": manufactured, "bowl": bowl, "tidal": tidal, "dambreak": dambreak}

@@ -226,6 +226,39 @@ typedef struct rkdg_quad_mesh_spec {
 typedef struct rkdg_mesh rkdg_mesh; /* owns the arrays a rkdg_mesh_desc points into */
 rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* spec, rkdg_mesh** out, rkdg_mesh_desc* desc);
 
+/*
+ * Ensembles: several independent meshes stepped together (no counterpart in the reference, which
+ * runs one mesh per process).  Instances of equal polynomial order are merged into one element
+ * range, each padded to whole 256-element tiles, and stepped by one stage launch per order;
+ * each keeps its own dt, time, ramp and forcing.  Instances of one order need equal Runge-Kutta
+ * scheme (at least 2 stages), gravity, friction, wetting-drying, limiter and problem options and
+ * one constituent list.  A batch context accepts rkdg_step, rkdg_sync, rkdg_reset_timers,
+ * rkdg_report and the rkdg_batch_* calls; instance state access needs a step boundary.
+ */
+typedef struct rkdg_schedule {
+    int32_t block;          /* work-items per block: 64, 128 or 256 */
+    int32_t groups_per_xcd; /* 0: one work-group per element tile; > 0: persistent stage kernel with
+                               this many work-groups on each of the 8 XCDs */
+    int32_t graph;          /* 1: each step replays one captured hipGraph (all stages, all orders) */
+} rkdg_schedule;
+
+rkdg_status rkdg_create_batch(int32_t n, const rkdg_mesh_desc* meshes, const rkdg_params* params,
+                              rkdg_precision precision, rkdg_ctx** out);
+int32_t rkdg_batch_size(const rkdg_ctx* ctx);
+rkdg_status rkdg_batch_get_state(rkdg_ctx* ctx, int32_t instance, double* q, uint8_t* wet, uint8_t* limited,
+                                 uint8_t* troubled);
+rkdg_status rkdg_batch_set_state(rkdg_ctx* ctx, int32_t instance, const double* q, const uint8_t* wet);
+rkdg_status rkdg_batch_get_time(rkdg_ctx* ctx, int32_t instance, double* t, int32_t* step, int32_t* stage);
+/* First NaN found (instance, element within it, step, stage), or all -1. */
+rkdg_status rkdg_batch_nan_info(rkdg_ctx* ctx, int32_t* instance, int32_t* element, int32_t* step, int32_t* stage);
+rkdg_status rkdg_batch_set_schedule(rkdg_ctx* ctx, const rkdg_schedule* schedule);
+rkdg_status rkdg_batch_get_schedule(const rkdg_ctx* ctx, rkdg_schedule* schedule);
+/* Times steps_per_trial steps under each candidate schedule (block 64/128/256 x one-group-per-
+ * tile or 1, 2, 4 persistent work-groups per CU x graph off/on) from the same state, keeps the
+ * fastest, and restores the state; best_ms_per_step may be NULL. */
+rkdg_status rkdg_batch_autotune(rkdg_ctx* ctx, int32_t steps_per_trial, rkdg_schedule* best,
+                                double* best_ms_per_step);
+
 /* Reads an ADCIRC mesh (.14) and, if bcis_path is not NULL, its boundary-condition /
  * interface-specialisation data (.bcis) into a mesh description.  Replaces AdcircFormat
  * (preprocessor/ADCIRC_reader/adcirc_format.cpp:3-130) with its boundary typing
