@@ -11,7 +11,7 @@ Newton reciprocals; gate 1e-12 relative L2 against FP64 plain) and FP32 bulk (ga
 value = DOF * stages / s of the headline form: the fastest form within its gate.  roofline: algorithmic HBM bytes of one stage launch / mean
 stage-kernel time from HIP events on the context's stream, against 8 TB/s.
 cpu_baseline: the FP64 CPU oracle (OpenMP over host cores) on a bounded sample of the same
-recipe, timed on this host.
+recipe, timed on this host.  ensemble: the 64-instance ensemble (FP32 bulk, autotuned schedule).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -87,6 +87,35 @@ def copy_probe(mib):
     log(f"copied {mib} MiB x 5 ({n * 4} bytes read and written per copy)")
 
 
+def ensemble_bench(steps, warmup, scale):
+    """64-instance ensemble (triangles log-uniform 20k-400k, p in {1, 2}, tide 0.1-1.5 m, the
+    tidal recipe), FP32 bulk, schedule picked by the autotuner, timed like the single mesh."""
+    from dgswemv2_amd import RKDG_FP32_BULK, Batch, workloads as W
+    log("building the 64-instance ensemble, scale", scale)
+    meshes, params, info = W.ensemble(n=64, scale=scale)
+    ne = sum(i["triangles"] for i in info)
+    dof = sum(i["triangles"] * (i["p"] + 1) * (i["p"] + 2) // 2 * 3 for i in info)
+    b = Batch(meshes, params, RKDG_FP32_BULK)
+    best, tuned_ms = b.autotune(steps_per_trial=3)
+    log("ensemble autotune", best, "ms/step", tuned_ms)
+    b.step(warmup)
+    b.sync()
+    b.reset_timers()
+    t0 = time.perf_counter()
+    b.step(steps)
+    b.sync()
+    wall = time.perf_counter() - t0
+    rep = b.report()
+    ns = params[0].nstages
+    ms_step = rep["total_ms"] / steps
+    achieved = rep["stage_bytes_alg"] * ns / (ms_step * 1e-3) / 1e9
+    b.close()
+    return {"instances": len(meshes), "triangles": ne, "p2_instances": sum(1 for i in info if i["p"] == 2),
+            "value": dof * ns * steps / wall, "unit": "DOF-stage updates/s", "ms_per_step": wall / steps * 1e3,
+            "schedule": best, "achieved_GBs": achieved, "roofline_frac": achieved / HBM_PEAK_GBS,
+            "dtype": "f32"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -94,7 +123,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--scale", type=float, default=1.0, help="mesh scale (1.0 = the 1M-triangle workload)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--form", choices=("all", "fp64_plain", "fp64_fast", "fp32_bulk"), default="all",
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-scale", type=float, default=1.0)
+    ap.add_argument("--form", choices=("all", "fp64_plain", "fp64_fast", "fp32_bulk", "none"), default="all",
                     help="time one form only (short profiled runs); the JSON line needs all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (counter calibration) and exit")
@@ -136,6 +167,8 @@ def main():
         }
         log(name, json.dumps(forms[name]))
     if args.form != "all":
+        if not args.no_ensemble:
+            log("ensemble", json.dumps(ensemble_bench(args.steps, args.warmup, args.ensemble_scale)))
         log("single form timed; no JSON line")
         return
     # errors against the FP64 plain form after the same steps from the same start state
@@ -161,6 +194,9 @@ def main():
         except Exception:
             traffic = None
 
+    ens = None if args.no_ensemble else ensemble_bench(args.steps, args.warmup, args.ensemble_scale)
+    if ens:
+        log("ensemble", json.dumps(ens))
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log("cpu baseline", json.dumps(cpu))
@@ -187,6 +223,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": h["achieved_GBs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": h["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

@@ -24,6 +24,9 @@
 #else
 #define RKDG_HD
 #endif
+// The per-element stage bodies are called from more than one kernel; they must inline into each
+// (otherwise the kernel argument block is passed through scratch memory).
+#define RKDG_FORCE_INLINE __attribute__((always_inline))
 
 
 #include <cmath>
@@ -94,22 +97,37 @@ struct StageArgs {
     int key_ns;
 };
 
-// Per-element view of the instance-dependent stage scalars (single mesh: the launch's own).
+// Per-element view of the instance-dependent stage scalars.  ENS (compile time): an ensemble,
+// whose time, ramp and dt * brk coefficients come from the per-instance tables; otherwise the
+// launch's own.  Never chosen at run time: a run-time choice between a table and the kernel's
+// argument block lets the compiler load through a selected pointer, which forces the whole
+// argument block into scratch memory.
 template <typename Real>
 struct InstScal {
     double t, ramp;
-    const Real* coef;  // null: use A.cur_c / A.term_c
+    const Real* coef;  // ensembles: cur_c, term_c[0..7] of the element's instance
 };
 
-// Instance of element e in an ensemble (uniform over a work-group's tile), or -1.
-template <typename Real>
-RKDG_HD inline int instance_of(const StageArgs<Real>& A, int e) {
-    return A.tile_inst ? A.tile_inst[e >> 6] : -1;
-}
-template <typename Real>
+template <bool ENS, typename Real>
 RKDG_HD inline InstScal<Real> inst_scal(const StageArgs<Real>& A, int inst) {
-    if (inst < 0) return InstScal<Real>{A.t, A.ramp, nullptr};
-    return InstScal<Real>{A.inst_tr[2 * inst], A.inst_tr[2 * inst + 1], A.inst_coef + 9 * (size_t)inst};
+    if constexpr (ENS)
+        return InstScal<Real>{A.inst_tr[2 * inst], A.inst_tr[2 * inst + 1], A.inst_coef + 9 * (size_t)inst};
+    else
+        return InstScal<Real>{A.t, A.ramp, nullptr};
+}
+template <bool ENS, typename Real>
+RKDG_HD inline Real cur_coef(const StageArgs<Real>& A, const InstScal<Real>& IS) {
+    if constexpr (ENS)
+        return IS.coef[0];
+    else
+        return A.cur_c;
+}
+template <bool ENS, typename Real>
+RKDG_HD inline Real term_coef(const StageArgs<Real>& A, const InstScal<Real>& IS, int i) {
+    if constexpr (ENS)
+        return IS.coef[1 + i];
+    else
+        return A.term_c[i];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -439,7 +457,7 @@ RKDG_HD inline Real harmonic_forcing(const StageArgs<Real>& A, int ed, int g, do
 
 // Wetting-drying of the new state (swe_post_wet_dry.hpp:5-145, FP64), store, NaN scan.
 template <typename Real, int P, bool WD>
-RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Real (&nq)[3][Sizes<P>::ND]) {
+RKDG_HD RKDG_FORCE_INLINE inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Real (&nq)[3][Sizes<P>::ND]) {
     constexpr int ND = Sizes<P>::ND;
     const int ne = A.ne;
     if (WD) {
@@ -487,8 +505,8 @@ RKDG_HD inline void finish_element(const StageArgs<Real>& A, int e, bool wet, Re
 // ---------------------------------------------------------------------------------------
 // The fused stage of one element, reference arithmetic order (FP64 plain form).
 
-template <typename Real, int P, bool WD, bool CF>
-RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e, const InstScal<Real>& IS) {
+template <typename Real, int P, bool WD, bool CF, bool ENS>
+RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e, const InstScal<Real>& IS) {
     using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
@@ -774,11 +792,10 @@ RKDG_HD inline void stage_element(const StageArgs<Real>& A, const FaceTables<Rea
             Real acc = Real(0.0);
             for (int i = 0; i < A.nterm; ++i) {
                 Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
-                if (A.term_sol[i])
-                    term = term + (IS.coef ? IS.coef[1 + i] : A.term_c[i]) * ldc(A.term_sol[i], ne, k * ND + d, e);
+                if (A.term_sol[i]) term = term + term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc = acc + term;
             }
-            acc = acc + (A.cur_a * q[k][d] + (IS.coef ? IS.coef[0] : A.cur_c) * sol);
+            acc = acc + (A.cur_a * q[k][d] + cur_coef<ENS>(A, IS) * sol);
             nq[k][d] = acc;
         }
     finish_element<Real, P, WD>(A, e, wet, nq);
@@ -950,8 +967,8 @@ RKDG_HD inline void exterior_state(const StageArgs<Real>& A, int bc, int e, int 
 template <typename Real>
 RKDG_HD inline Real fmac_tab(Real a, Real c, Real r) { return c != Real(0) ? fmaR(a, c, r) : r; }
 
-template <typename Real, int P, bool WD, bool CF>
-RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phie_lds, const Real* phiew_lds,
+template <typename Real, int P, bool WD, bool CF, bool ENS>
+RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& A, const Real* phie_lds, const Real* phiew_lds,
                                        int e, const InstScal<Real>& IS) {
     using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
@@ -1257,10 +1274,10 @@ RKDG_HD inline void stage_element_fast(const StageArgs<Real>& A, const Real* phi
         for (int d = 0; d < ND; ++d) {
             const Real sol = rhs[k][d] * C::minv[0][d];
             if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
-            Real acc = A.cur_a * q[k][d] + (IS.coef ? IS.coef[0] : A.cur_c) * sol;
+            Real acc = A.cur_a * q[k][d] + cur_coef<ENS>(A, IS) * sol;
             for (int i = 0; i < A.nterm; ++i) {
                 Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
-                if (A.term_sol[i]) term += (IS.coef ? IS.coef[1 + i] : A.term_c[i]) * ldc(A.term_sol[i], ne, k * ND + d, e);
+                if (A.term_sol[i]) term += term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc += term;
             }
             nq[k][d] = acc;

@@ -56,7 +56,7 @@ RKDG_HD inline void load_linear(const LimiterArgs<Real>& L, int e, double ql[3][
 }
 
 template <typename Real, int P>
-RKDG_HD inline void sl_prepare_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
+RKDG_HD RKDG_FORCE_INLINE inline void sl_prepare_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
     constexpr int ND = Sizes<P>::ND, NE = Sizes<P>::NE;
     const int ne = L.ne;
     const bool wet = L.wet[e] != 0;
@@ -145,7 +145,7 @@ RKDG_HD inline void mv3(const double A[3][3], const double* x, double* y) {
 }
 
 template <typename Real, int P>
-RKDG_HD inline void sl_limit_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
+RKDG_HD RKDG_FORCE_INLINE inline void sl_limit_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
     constexpr int ND = Sizes<P>::ND;
     const int ne = L.ne;
     const bool wet = L.wet[e] != 0;
