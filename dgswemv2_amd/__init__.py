@@ -17,7 +17,7 @@ from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "read_adcirc", "make_params", "Simulation", "Batch", "RkdgError", "ndof_of",
+    "Mesh", "quad_mesh", "read_adcirc", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
@@ -263,6 +263,98 @@ class Simulation:
         n = int(math.ceil(run_time / self.params.dt))
         self.step(n)
         return n
+
+    # ---- learned troubled-cell indicator
+    def set_indicator(self, flat=None, threshold=0.0, judge=False):
+        """Gate the limiter with the MLP of flat parameters (w1 b1 w2 b2 w3 b3 w4 b4); None removes it."""
+        if flat is None:
+            self._check(self.lib.rkdg_set_indicator(self.ctx, None, 0))
+            return
+        w = N.weights_from_flat(flat, threshold)
+        self._check(self.lib.rkdg_set_indicator(self.ctx, C.byref(w), 1 if judge else 0))
+
+    def indicator_stats(self, reset=False):
+        st = N.IndicatorStats()
+        self._check(self.lib.rkdg_get_indicator_stats(self.ctx, C.byref(st), 1 if reset else 0))
+        return {f: getattr(st, f) for f, _ in N.IndicatorStats._fields_}
+
+    def capture(self, on=True):
+        """Record features / labels (/ logits) of each limiter stage's state before limiting."""
+        self._check(self.lib.rkdg_capture(self.ctx, 1 if on else 0))
+
+    def get_capture(self, logits=False):
+        X = np.empty((N.MLP_IN, self.ne), np.float32)
+        lab = np.empty(self.ne, np.uint8)
+        ok = np.empty(self.ne, np.uint8)
+        lg = np.empty(self.ne, np.float32) if logits else None
+        ch = np.empty(self.ne, np.float32)
+        self._check(self.lib.rkdg_get_capture(self.ctx, N.fptr(X), N.uptr(lab), N.uptr(ok), N.fptr(lg), N.fptr(ch)))
+        out = {"features": X, "label": lab, "may_act": ok, "change": ch}
+        if logits:
+            out["logit"] = lg
+        return out
+
+
+class MlpTrainer:
+    """The indicator's network trained on the GPU (include/rkdg.h rkdg_mlp_*)."""
+
+    def __init__(self, flat, capacity, threshold=0.0):
+        self.lib = N.load_library()
+        w = N.weights_from_flat(flat, threshold)
+        h = C.c_void_p()
+        st = self.lib.rkdg_mlp_trainer_create(C.byref(w), int(capacity), C.byref(h))
+        if st != N.RKDG_OK:
+            raise RkdgError(st, "trainer create failed")
+        self.h, self.capacity = h, int(capacity)
+
+    def _check(self, st):
+        if st != N.RKDG_OK:
+            raise RkdgError(st, self.lib.rkdg_mlp_trainer_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.rkdg_mlp_trainer_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_data(self, X, y):
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32).reshape(-1)
+        if X.shape != (N.MLP_IN, y.size):
+            raise ValueError("X must be [36][n] with n = len(y)")
+        self._check(self.lib.rkdg_mlp_set_data(self.h, N.fptr(X), N.fptr(y), y.size))
+
+    def train(self, epochs, batch=1024, lr=2e-3, beta1=0.9, beta2=0.999, eps=1e-8, pos_weight=1.0, seed=5):
+        prm = N.TrainParams(batch, epochs, lr, beta1, beta2, eps, pos_weight, seed)
+        losses = np.empty(max(1, epochs), np.float32)
+        self._check(self.lib.rkdg_mlp_train(self.h, C.byref(prm), N.fptr(losses)))
+        return losses[:epochs]
+
+    def weights(self):
+        w = N.MlpWeights()
+        self._check(self.lib.rkdg_mlp_get_weights(self.h, C.byref(w)))
+        return N.weights_to_flat(w)
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, np.float32)
+        out = np.empty(X.shape[1], np.float32)
+        self._check(self.lib.rkdg_mlp_forward(self.h, N.fptr(X), X.shape[1], N.fptr(out)))
+        return out
+
+    def gradient(self, X, y, pos_weight=1.0):
+        """(loss, flat gradient) of one batch under the current weights."""
+        X = np.ascontiguousarray(X, np.float32)
+        y = np.ascontiguousarray(y, np.float32).reshape(-1)
+        loss = C.c_float()
+        g = N.MlpWeights()
+        self._check(self.lib.rkdg_mlp_gradient(self.h, N.fptr(X), N.fptr(y), y.size, pos_weight, C.byref(loss),
+                                               C.byref(g)))
+        return loss.value, N.weights_to_flat(g)
 
 
 class Batch:

@@ -80,6 +80,35 @@ class QuadMeshSpec(C.Structure):
     ]
 
 
+MLP_IN, MLP_HIDDEN = 36, 64
+
+
+class MlpWeights(C.Structure):
+    _fields_ = [
+        ("w1", C.c_float * (MLP_HIDDEN * MLP_IN)), ("b1", C.c_float * MLP_HIDDEN),
+        ("w2", C.c_float * (MLP_HIDDEN * MLP_HIDDEN)), ("b2", C.c_float * MLP_HIDDEN),
+        ("w3", C.c_float * (MLP_HIDDEN * MLP_HIDDEN)), ("b3", C.c_float * MLP_HIDDEN),
+        ("w4", C.c_float * MLP_HIDDEN), ("b4", C.c_float), ("threshold", C.c_float),
+    ]
+
+
+class IndicatorStats(C.Structure):
+    _fields_ = [
+        ("stages", C.c_int64), ("candidates", C.c_int64), ("flagged", C.c_int64), ("write_backs", C.c_int64),
+        ("significant", C.c_int64), ("missed", C.c_int64), ("missed_significant", C.c_int64),
+        ("max_missed_change", C.c_double),
+    ]
+
+
+class TrainParams(C.Structure):
+    _fields_ = [
+        ("batch", C.c_int32), ("epochs", C.c_int32), ("lr", C.c_float), ("beta1", C.c_float),
+        ("beta2", C.c_float), ("eps", C.c_float), ("pos_weight", C.c_float), ("seed", C.c_uint64),
+    ]
+
+
+_fp = C.POINTER(C.c_float)
+
 # Every entry point declared in include/rkdg.h, with its ctypes signature.
 SIGNATURES = {
     "rkdg_create": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(Params), C.c_int, C.POINTER(C.c_void_p)]),
@@ -112,6 +141,18 @@ SIGNATURES = {
     "rkdg_batch_set_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "rkdg_batch_get_schedule": (C.c_int, [C.c_void_p, C.POINTER(Schedule)]),
     "rkdg_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Schedule), _dp]),
+    "rkdg_set_indicator": (C.c_int, [C.c_void_p, C.POINTER(MlpWeights), C.c_int32]),
+    "rkdg_get_indicator_stats": (C.c_int, [C.c_void_p, C.POINTER(IndicatorStats), C.c_int32]),
+    "rkdg_capture": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rkdg_get_capture": (C.c_int, [C.c_void_p, _fp, _up, _up, _fp, _fp]),
+    "rkdg_mlp_trainer_create": (C.c_int, [C.POINTER(MlpWeights), C.c_int32, C.POINTER(C.c_void_p)]),
+    "rkdg_mlp_set_data": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32]),
+    "rkdg_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(TrainParams), _fp]),
+    "rkdg_mlp_get_weights": (C.c_int, [C.c_void_p, C.POINTER(MlpWeights)]),
+    "rkdg_mlp_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "rkdg_mlp_gradient": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.c_float, _fp, C.POINTER(MlpWeights)]),
+    "rkdg_mlp_trainer_error": (C.c_char_p, [C.c_void_p]),
+    "rkdg_mlp_trainer_destroy": (None, [C.c_void_p]),
 }
 
 _lib = None
@@ -145,6 +186,27 @@ def iptr(a):
 
 def uptr(a):
     return a.ctypes.data_as(_up) if a is not None else None
+
+
+def fptr(a):
+    return a.ctypes.data_as(_fp) if a is not None else None
+
+
+def weights_from_flat(flat, threshold=0.0):
+    """MlpWeights from the flat parameter vector (w1 b1 w2 b2 w3 b3 w4 b4)."""
+    flat = np.ascontiguousarray(flat, np.float32).reshape(-1)
+    n = C.sizeof(MlpWeights) // 4 - 1
+    if flat.size != n:
+        raise ValueError(f"expected {n} parameters, got {flat.size}")
+    w = MlpWeights()
+    C.memmove(C.addressof(w), flat.ctypes.data, 4 * n)
+    w.threshold = threshold
+    return w
+
+
+def weights_to_flat(w):
+    n = C.sizeof(MlpWeights) // 4 - 1
+    return np.frombuffer(bytes(w), np.float32, count=n).copy()
 
 
 class RkdgError(RuntimeError):

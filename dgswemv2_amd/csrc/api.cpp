@@ -45,6 +45,11 @@ struct EngineBase {
     virtual rkdg_status inst_set(int i, const double* q, const uint8_t* wet) = 0;
     virtual rkdg_status inst_time(int i, double* t, int* step, int* stage) = 0;
     virtual rkdg_status snapshot(bool restore) = 0;  // save / restore the step-boundary state
+    // learned troubled-cell indicator
+    virtual rkdg_status set_indicator(const rkdg_mlp_weights* w, int judge) = 0;
+    virtual rkdg_status indicator_stats(rkdg_indicator_stats* st, bool reset) = 0;
+    virtual rkdg_status set_capture(bool on) = 0;
+    virtual rkdg_status get_capture(float* feat, uint8_t* label, uint8_t* ok, float* logit, float* change) = 0;
     int block = 128;            // work-items per block (measured best for the stage kernels on 1M elements)
     int persistent_grid = 0;    // > 0: persistent stage kernel on this many work-groups (ensembles)
     bool capturing = false;     // inside a graph capture: no host-side event records
@@ -80,6 +85,15 @@ struct Engine final : EngineBase {
     Real* coef_d = nullptr;
     unsigned long long* step_ctr_d = nullptr;
     std::vector<void*> snap;  // snapshot buffers (autotuning)
+    // learned indicator: packed bf16 weights, vectors, judge counters, capture buffers
+    bool ind_on = false, ind_judge = false, cap_on = false, cap_valid = false, cap_logits = false;
+    uint16_t* ind_pack = nullptr;
+    float* ind_vec = nullptr;
+    unsigned long long* ind_cnt = nullptr;
+    int *ind_list = nullptr, *ind_count = nullptr;
+    int64_t ind_stages = 0;
+    float *cap_feat = nullptr, *cap_logit = nullptr, *cap_change = nullptr;
+    uint8_t *cap_label = nullptr, *cap_ok = nullptr;
     // timing
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> stage_ev, lim_ev;
@@ -216,6 +230,125 @@ struct Engine final : EngineBase {
                fail("hipMemset", hipGetLastError());
     }
 
+    rkdg::GateArgs<Real> gate_args(const rkdg::LimiterArgs<Real>& L) const {
+        rkdg::GateArgs<Real> G;
+        std::memset(&G, 0, sizeof G);
+        G.ne = M.ne, G.q = L.q, G.wet = L.wet, G.neighbor = neighbor, G.bath_bc = bath_bc, G.g = M.prm.g;
+        if (cap_on) G.feat = cap_feat, G.ok = cap_ok;
+        return G;
+    }
+    rkdg::JudgeArgs judge_args() const {
+        rkdg::JudgeArgs J;
+        std::memset(&J, 0, sizeof J);
+        J.h_floor = rkdg::mlp::H_FLOOR, J.tol = sizeof(Real) == 4 ? rkdg::mlp::SIG_TOL_FP32 : rkdg::mlp::SIG_TOL;
+        if (cap_on) J.label = cap_label, J.change = cap_change;
+        return J;
+    }
+    // reference limiter: trouble check, dilation, limit (with capture: features and labels of the
+    // state before the limit pass)
+    hipError_t plain_limiter(const rkdg::LimiterArgs<Real>& L) {
+        hipError_t e = rkdg::launch_limiter_prepare<Real>(M.p, L, stream, block);
+        if (e == hipSuccess && cap_on) {
+            e = rkdg::launch_gate<Real>(M.p, gate_args(L), false, stream);
+            if (e == hipSuccess) e = rkdg::launch_limiter_judge<Real>(M.p, L, judge_args(), stream, block);
+            cap_valid = true, cap_logits = false;
+        }
+        if (e == hipSuccess) e = rkdg::launch_limiter_limit<Real>(M.p, L, stream, block);
+        return e;
+    }
+    // learned form: the gate writes the barycentres, the troubled (gate) flags and the list of
+    // open elements; the limit pass runs over that list
+    hipError_t learned_limiter(const rkdg::LimiterArgs<Real>& L) {
+        rkdg::GateArgs<Real> G = gate_args(L);
+        G.wpack = ind_pack, G.vec = ind_vec, G.baryctr = baryctr, G.flag = troubled;
+        G.limited = limited, G.list = ind_list, G.list_count = ind_count;
+        if (cap_on) G.logit = cap_logit;
+        hipError_t e = hipMemsetAsync(ind_count, 0, sizeof(int), stream);
+        if (e == hipSuccess) e = rkdg::launch_gate<Real>(M.p, G, true, stream);
+        if (e == hipSuccess && (ind_judge || cap_on)) {
+            rkdg::JudgeArgs J = judge_args();
+            J.flag = troubled;
+            if (ind_judge) J.cnt = ind_cnt;
+            e = rkdg::launch_limiter_judge<Real>(M.p, L, J, stream, block);
+        }
+        if (cap_on) cap_valid = true, cap_logits = true;
+        if (e == hipSuccess) e = rkdg::launch_limiter_list<Real>(M.p, L, ind_list, ind_count, stream);
+        ++ind_stages;
+        return e;
+    }
+
+    rkdg_status set_indicator(const rkdg_mlp_weights* w, int judge) override {
+        if (!w) {
+            ind_on = false;
+            return RKDG_OK;
+        }
+        if (!M.prm.slope_limiting) return fail("the indicator needs the slope limiter on", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (M.ens.n > 0) return fail("no indicator for ensembles", hipErrorInvalidValue), RKDG_ERR_ARG;
+        namespace ml = rkdg::mlp;
+        const float* theta = reinterpret_cast<const float*>(w);
+        std::vector<uint16_t> pack(3 * (size_t)ml::PACK);
+        for (int m = 0; m < 3; ++m)
+            for (int i = 0; i < ml::PACK; ++i) {
+                const int src = ml::pack_source(m, i);
+                pack[(size_t)m * ml::PACK + i] = src < 0 ? (uint16_t)0 : ml::bf16_bits(theta[src]);
+            }
+        std::vector<float> vec(ml::NVEC);
+        std::copy(w->b1, w->b1 + 64, vec.begin());
+        std::copy(w->b2, w->b2 + 64, vec.begin() + 64);
+        std::copy(w->b3, w->b3 + 64, vec.begin() + 128);
+        std::copy(w->w4, w->w4 + 64, vec.begin() + 192);
+        vec[256] = w->b4, vec[257] = w->threshold;
+        if (!ind_pack && (!alloc(ind_pack, pack.size()) || !alloc(ind_vec, vec.size()) || !alloc(ind_cnt, 8) ||
+                          !alloc(ind_list, (size_t)M.ne) || !alloc(ind_count, 1)))
+            return RKDG_ERR_HIP;
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipMemcpy(ind_pack, pack.data(), pack.size() * sizeof(uint16_t), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(ind_vec, vec.data(), vec.size() * sizeof(float), hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemset(ind_cnt, 0, 8 * sizeof(unsigned long long));
+        if (e != hipSuccess) return fail("set_indicator", e), RKDG_ERR_HIP;
+        ind_on = true, ind_judge = judge != 0, ind_stages = 0;
+        return RKDG_OK;
+    }
+    rkdg_status indicator_stats(rkdg_indicator_stats* st, bool reset) override {
+        std::memset(st, 0, sizeof *st);
+        if (!ind_cnt) return RKDG_OK;
+        unsigned long long c[8];
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipMemcpy(c, ind_cnt, sizeof c, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && reset) e = hipMemset(ind_cnt, 0, sizeof c);
+        if (e != hipSuccess) return fail("indicator stats", e), RKDG_ERR_HIP;
+        st->stages = ind_stages;
+        st->candidates = (int64_t)c[0], st->flagged = (int64_t)c[1], st->write_backs = (int64_t)c[2];
+        st->significant = (int64_t)c[3], st->missed = (int64_t)c[4], st->missed_significant = (int64_t)c[5];
+        std::memcpy(&st->max_missed_change, &c[6], sizeof(double));
+        if (reset) ind_stages = 0;
+        return RKDG_OK;
+    }
+    rkdg_status set_capture(bool on) override {
+        if (on && !M.prm.slope_limiting) return fail("capture needs the slope limiter on", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (on && M.ens.n > 0) return fail("no capture for ensembles", hipErrorInvalidValue), RKDG_ERR_ARG;
+        const size_t ne = M.ne;
+        if (on && !cap_feat &&
+            (!alloc(cap_feat, rkdg::mlp::NIN * ne) || !alloc(cap_logit, ne) || !alloc(cap_label, ne) || !alloc(cap_ok, ne) ||
+             !alloc(cap_change, ne)))
+            return RKDG_ERR_HIP;
+        cap_on = on;
+        return RKDG_OK;
+    }
+    rkdg_status get_capture(float* feat, uint8_t* label, uint8_t* ok, float* logit, float* change) override {
+        if (!cap_valid) return fail("nothing captured yet", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (logit && !cap_logits) return fail("the last captured stage ran without an indicator", hipErrorInvalidValue), RKDG_ERR_ARG;
+        const size_t ne = M.ne;
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess && feat) e = hipMemcpy(feat, cap_feat, rkdg::mlp::NIN * ne * sizeof(float), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && label) e = hipMemcpy(label, cap_label, ne, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && ok) e = hipMemcpy(ok, cap_ok, ne, hipMemcpyDeviceToHost);
+        if (e == hipSuccess && logit) e = hipMemcpy(logit, cap_logit, ne * sizeof(float), hipMemcpyDeviceToHost);
+        if (e == hipSuccess && change) e = hipMemcpy(change, cap_change, ne * sizeof(float), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail("get_capture", e), RKDG_ERR_HIP;
+        return RKDG_OK;
+    }
+
     rkdg_status stage() override {
         rkdg::Stepper& st = M.stepper;
         const int s = st.stage, ns = st.nstages;
@@ -255,7 +388,7 @@ struct Engine final : EngineBase {
         A.problem = M.prm.problem;
         for (int i = 0; i < 4; ++i) A.pparam[i] = M.prm.problem_param[i];
         A.h_o = M.h_o, A.h_thr = M.h_thr;
-        A.nan_check = M.prm.slope_limiting ? 0 : 1;
+        A.nan_check = (M.prm.slope_limiting && !ind_on) ? 0 : 1;  // the gated limiter scans only what it limits
         A.nan_key = d_nan;
         A.stage_key = (unsigned long long)st.step * ns + s;
         if (ens) {
@@ -300,7 +433,7 @@ struct Engine final : EngineBase {
             if (ens) L.step_ctr = step_ctr_d, L.key_ns = ns;
             hipEvent_t l0 = nullptr, l1 = nullptr;
             if (tm) l0 = take_event(), hipEventRecord(l0, stream);
-            e = rkdg::launch_limiter<Real>(M.p, L, stream, block);
+            e = ind_on ? learned_limiter(L) : plain_limiter(L);
             if (e != hipSuccess) return fail("limiter kernel launch", e), RKDG_ERR_HIP;
             if (tm) l1 = take_event(), hipEventRecord(l1, stream), lim_ev.push_back({l0, l1});
         }
@@ -1044,6 +1177,39 @@ rkdg_status rkdg_batch_autotune(rkdg_ctx* ctx, int32_t steps_per_trial, rkdg_sch
     if (best) *best = win;
     if (best_ms) *best_ms = win_ms;
     return RKDG_OK;
+}
+
+rkdg_status rkdg_set_indicator(rkdg_ctx* ctx, const rkdg_mlp_weights* w, int32_t judge) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no learned indicator");
+    rkdg_status st = ctx->eng->set_indicator(w, judge);
+    if (st != RKDG_OK) ctx->err = ctx->eng->err;
+    return st;
+}
+
+rkdg_status rkdg_get_indicator_stats(rkdg_ctx* ctx, rkdg_indicator_stats* stats, int32_t reset) {
+    if (!ctx || !stats) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no learned indicator");
+    rkdg_status st = ctx->eng->indicator_stats(stats, reset != 0);
+    if (st != RKDG_OK) ctx->err = ctx->eng->err;
+    return st;
+}
+
+rkdg_status rkdg_capture(rkdg_ctx* ctx, int32_t on) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no capture");
+    rkdg_status st = ctx->eng->set_capture(on != 0);
+    if (st != RKDG_OK) ctx->err = ctx->eng->err;
+    return st;
+}
+
+rkdg_status rkdg_get_capture(rkdg_ctx* ctx, float* features, uint8_t* label, uint8_t* may_act, float* logit,
+                             float* change) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no capture");
+    rkdg_status st = ctx->eng->get_capture(features, label, may_act, logit, change);
+    if (st != RKDG_OK) ctx->err = ctx->eng->err;
+    return st;
 }
 
 rkdg_status rkdg_master_tables(int32_t p, int32_t* ndof, int32_t* ngp, double* phi, double* dz1, double* dz2,

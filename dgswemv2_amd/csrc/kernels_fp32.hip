@@ -5,4 +5,8 @@
 namespace rkdg {
 template hipError_t launch_stage<float, true>(int, bool, bool, const StageArgs<float>&, hipStream_t, int, int);
 template hipError_t launch_limiter<float>(int, const LimiterArgs<float>&, hipStream_t, int);
+template hipError_t launch_limiter_prepare<float>(int, const LimiterArgs<float>&, hipStream_t, int);
+template hipError_t launch_limiter_limit<float>(int, const LimiterArgs<float>&, hipStream_t, int);
+template hipError_t launch_limiter_list<float>(int, const LimiterArgs<float>&, const int*, const int*, hipStream_t);
+template hipError_t launch_limiter_judge<float>(int, const LimiterArgs<float>&, const JudgeArgs&, hipStream_t, int);
 }  // namespace rkdg

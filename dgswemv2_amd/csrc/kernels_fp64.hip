@@ -5,6 +5,10 @@
 namespace rkdg {
 template hipError_t launch_stage<double, false>(int, bool, bool, const StageArgs<double>&, hipStream_t, int, int);
 template hipError_t launch_limiter<double>(int, const LimiterArgs<double>&, hipStream_t, int);
+template hipError_t launch_limiter_prepare<double>(int, const LimiterArgs<double>&, hipStream_t, int);
+template hipError_t launch_limiter_limit<double>(int, const LimiterArgs<double>&, hipStream_t, int);
+template hipError_t launch_limiter_list<double>(int, const LimiterArgs<double>&, const int*, const int*, hipStream_t);
+template hipError_t launch_limiter_judge<double>(int, const LimiterArgs<double>&, const JudgeArgs&, hipStream_t, int);
 }  // namespace rkdg
 
 namespace rkdg {

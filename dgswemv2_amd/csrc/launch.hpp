@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "engine.hpp"
+#include "indicator.hpp"
 #include "limiter.hpp"
 
 namespace rkdg {
@@ -17,6 +18,28 @@ hipError_t launch_stage(int p, bool wd, bool cf, const StageArgs<Real>& A, hipSt
 
 template <typename Real>
 hipError_t launch_limiter(int p, const LimiterArgs<Real>& L, hipStream_t s, int block);
+// The limiter in two parts: barycentres + trouble check + its six dilation passes, then the
+// limit pass; and the learned form's limit pass over the gate's list of open elements.
+template <typename Real>
+hipError_t launch_limiter_prepare(int p, const LimiterArgs<Real>& L, hipStream_t s, int block);
+template <typename Real>
+hipError_t launch_limiter_limit(int p, const LimiterArgs<Real>& L, hipStream_t s, int block);
+template <typename Real>
+hipError_t launch_limiter_list(int p, const LimiterArgs<Real>& L, const int* list, const int* count, hipStream_t s);
+// Dry run of the limit pass over every element: labels and the judge's counters.
+template <typename Real>
+hipError_t launch_limiter_judge(int p, const LimiterArgs<Real>& L, const JudgeArgs& J, hipStream_t s, int block);
+
+// Learned indicator (indicator.hip): features (+ inference with mlp_on) over one state.
+template <typename Real>
+hipError_t launch_gate(int p, const GateArgs<Real>& G, bool mlp_on, hipStream_t s);
+// Trainer steps (indicator.hip).
+hipError_t launch_mlp_fwdbwd(const TrainArgs& T, bool backward, hipStream_t s);
+hipError_t launch_mlp_wgrad(const TrainArgs& T, float* part, float* loss_part, hipStream_t s);
+hipError_t launch_mlp_update(int nslices, const float* part, const float* loss_part, float* grad, float* theta,
+                             float* m1, float* m2, float lr, float b1, float b2, float eps, float c1, float c2,
+                             bool update, float* loss_out, hipStream_t s);
+hipError_t launch_mlp_pack(const float* theta, uint16_t* wpack, hipStream_t s);
 
 // Ensemble helpers (kernels_fp64.hip): per-instance stepper advance on the device, byte copy.
 hipError_t launch_batch_advance(int n, int ns, double* t, const double* dt, const double* rd, int* step, int* stage,

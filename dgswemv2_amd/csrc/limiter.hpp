@@ -1,5 +1,8 @@
 // Cockburn-Shu slope limiter and trouble check, per element (FP64 arithmetic in every form).
 //
+// The arithmetic is FP64 without fused multiply-adds in every form (each function turns contraction
+// off, whatever the translation unit's setting), so a limiter decision depends only on the state.
+//
 // Three passes replace CS_slope_limiter_serial (problem_slope_limiter/swe_CS_sl_serial.hpp:9-23):
 //   sl_prepare_element  - barycentre of the linear part (slope_limiting_prepare_element_kernel,
 //                         swe_CS_slope_limiter.hpp:8-20) and the KXRCF-style indicator I of
@@ -47,6 +50,7 @@ struct LimiterArgs {
 
 template <typename Real, int P>
 RKDG_HD inline void load_linear(const LimiterArgs<Real>& L, int e, double ql[3][3]) {
+#pragma clang fp contract(off)
     constexpr int ND = Sizes<P>::ND;
     for (int k = 0; k < 3; ++k) {
         const double m[3] = {(double)L.q[(size_t)(k * ND) * L.ne + e], (double)L.q[(size_t)(k * ND + 1) * L.ne + e],
@@ -57,6 +61,7 @@ RKDG_HD inline void load_linear(const LimiterArgs<Real>& L, int e, double ql[3][
 
 template <typename Real, int P>
 RKDG_HD RKDG_FORCE_INLINE inline void sl_prepare_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
+#pragma clang fp contract(off)
     constexpr int ND = Sizes<P>::ND, NE = Sizes<P>::NE;
     const int ne = L.ne;
     const bool wet = L.wet[e] != 0;
@@ -141,13 +146,26 @@ RKDG_HD inline void sl_dilate_element(int ne, const int* neighbor, const uint8_t
 }
 
 RKDG_HD inline void mv3(const double A[3][3], const double* x, double* y) {
+#pragma clang fp contract(off)
     for (int i = 0; i < 3; ++i) y[i] = (A[i][0] * x[0] + A[i][1] * x[1]) + A[i][2] * x[2];
 }
 
-template <typename Real, int P>
-RKDG_HD RKDG_FORCE_INLINE inline void sl_limit_element(const LimiterArgs<Real>& L, const FaceTables<Real>& FT, int e) {
+// What slope_limiting_kernel decides for one element: may act (element and neighbours wet), the
+// variables it writes back, and the 2-norm of each variable's vertex-value change.
+struct SlOutcome {
+    bool ok;
+    unsigned mask;
+    double del[3];
+};
+
+// WRITE: the limiter itself (writes the limited modes, the flags, the NaN scan); otherwise a dry
+// run that only reports the decision (the learned indicator's judge and label pass).
+template <typename Real, int P, bool WRITE>
+RKDG_HD RKDG_FORCE_INLINE inline SlOutcome sl_limit_core(const LimiterArgs<Real>& L, int e) {
+#pragma clang fp contract(off)
     constexpr int ND = Sizes<P>::ND;
     const int ne = L.ne;
+    SlOutcome out{false, 0u, {0.0, 0.0, 0.0}};
     const bool wet = L.wet[e] != 0;
     bool wet_neigh = true;
     int nbr[3];
@@ -157,6 +175,7 @@ RKDG_HD RKDG_FORCE_INLINE inline void sl_limit_element(const LimiterArgs<Real>& 
         wet_neigh = wet_neigh && w;
     }
     uint8_t lim[3] = {0, 0, 0};
+    out.ok = wet && wet_neigh;
     if (wet && wet_neigh) {
         using std::copysign;
         using std::fabs;
@@ -228,15 +247,20 @@ RKDG_HD RKDG_FORCE_INLINE inline void sl_limit_element(const LimiterArgs<Real>& 
             const double d0 = qv[0] - ql[k][0], d1 = qv[1] - ql[k][1], d2 = qv[2] - ql[k][2];
             const double del_norm = sqrt((d0 * d0 + d1 * d1) + d2 * d2);
             const double q_norm = sqrt((ql[k][0] * ql[k][0] + ql[k][1] * ql[k][1]) + ql[k][2] * ql[k][2]);
+            out.del[k] = del_norm;
             if (del_norm / q_norm > 1.0e-12) {
                 lim[k] = 1;
-                double m[3];
-                from_vertices(qv, m);
-                for (int d = 0; d < 3; ++d) L.q[(size_t)(k * ND + d) * ne + e] = (Real)m[d];
-                for (int d = 3; d < ND; ++d) L.q[(size_t)(k * ND + d) * ne + e] = Real(0.0);
+                out.mask |= 1u << k;
+                if constexpr (WRITE) {
+                    double m[3];
+                    from_vertices(qv, m);
+                    for (int d = 0; d < 3; ++d) L.q[(size_t)(k * ND + d) * ne + e] = (Real)m[d];
+                    for (int d = 3; d < ND; ++d) L.q[(size_t)(k * ND + d) * ne + e] = Real(0.0);
+                }
             }
         }
     }
+    if constexpr (!WRITE) return out;
     for (int k = 0; k < 3; ++k) L.limited[(size_t)k * ne + e] = lim[k];
     if (L.nan_check) {
         bool has_nan = false;
@@ -254,6 +278,31 @@ RKDG_HD RKDG_FORCE_INLINE inline void sl_limit_element(const LimiterArgs<Real>& 
 #endif
         }
     }
+    return out;
+}
+
+template <typename Real, int P>
+RKDG_HD RKDG_FORCE_INLINE inline void sl_limit_element(const LimiterArgs<Real>& L, const FaceTables<Real>&, int e) {
+    (void)sl_limit_core<Real, P, true>(L, e);
+}
+
+// Relative size of a write-back for the indicator's labels: the change of each variable's vertex
+// values over that variable's physical scale (h_s for ze, h_s sqrt(g h_s) for qx, qy, with
+// h_s = max(ze mode 0 + bathymetry at the barycentre, 0.01)); the largest over written-back variables.
+template <typename Real>
+RKDG_HD inline double sl_change(const LimiterArgs<Real>& L, const SlOutcome& o, int e, double h_floor) {
+#pragma clang fp contract(off)
+    double h = (double)L.q[e] + L.bath_bc[e];
+    h = h > h_floor ? h : h_floor;
+    using std::sqrt;
+    const double sq = h * sqrt(L.g * h);
+    double r = 0.0;
+    for (int k = 0; k < 3; ++k)
+        if (o.mask & (1u << k)) {
+            const double c = o.del[k] / (k == 0 ? h : sq);
+            r = c > r ? c : r;
+        }
+    return r;
 }
 
 }  // namespace rkdg

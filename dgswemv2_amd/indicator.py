@@ -1,0 +1,92 @@
+"""Learned troubled-cell indicator: training-data collection, GPU training, threshold choice.
+
+The network, its features and its kernels are described in include/rkdg.h and
+dgswemv2_amd/csrc/indicator.hpp.  The workflow here:
+
+  1. run the simulation with the plain limiter and capture on (Simulation.capture): every
+     limiter stage records the features of its state before limiting and the label (the plain
+     limiter writes back a change above 1e-8 of the element's physical scale);
+  2. train the MLP on the GPU (MlpTrainer: BF16-MFMA forward / back-propagation, Adam);
+  3. pick the gate threshold on held-out samples so that a target share of the positive
+     samples (recall) opens the gate;
+  4. Simulation.set_indicator(weights, threshold): the limiter runs only where the gate opens.
+Only elements where the limiter may act (element and neighbours wet) are training samples;
+elsewhere the gate is closed by construction.
+"""
+import numpy as np
+
+from . import MlpTrainer
+from ._native import MLP_HIDDEN, MLP_IN
+
+NPARAM = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + 2 * (MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN) + MLP_HIDDEN + 1
+
+
+def init_weights(seed=5):
+    """He-initialised weights, zero hidden biases, output bias -2 (positives are the minority)."""
+    rng = np.random.default_rng(seed)
+    parts = [rng.normal(0, np.sqrt(2.0 / MLP_IN), MLP_HIDDEN * MLP_IN), np.zeros(MLP_HIDDEN),
+             rng.normal(0, np.sqrt(2.0 / MLP_HIDDEN), MLP_HIDDEN * MLP_HIDDEN), np.zeros(MLP_HIDDEN),
+             rng.normal(0, np.sqrt(2.0 / MLP_HIDDEN), MLP_HIDDEN * MLP_HIDDEN), np.zeros(MLP_HIDDEN),
+             rng.normal(0, np.sqrt(1.0 / MLP_HIDDEN), MLP_HIDDEN), np.array([-2.0])]
+    return np.concatenate(parts).astype(np.float32)
+
+
+def collect(sim, steps, every=1, neg_keep=1.0, seed=0):
+    """Steps `sim` (plain limiter) `steps` full steps with capture on and keeps the samples of
+    every `every`-th limiter stage where the limiter may act; negatives are kept with
+    probability neg_keep.  Returns (X [36][n] float32, y [n] float32, stages sampled)."""
+    rng = np.random.default_rng(seed)
+    ns = sim.params.nstages
+    sim.capture(True)
+    Xs, ys, k = [], [], 0
+    for _ in range(steps * ns):
+        sim.stage()
+        k += 1
+        if k % every:
+            continue
+        c = sim.get_capture()
+        m = c["may_act"] != 0
+        y = c["label"][m].astype(np.float32)
+        X = c["features"][:, m]
+        if neg_keep < 1.0:
+            keep = (y > 0) | (rng.uniform(size=y.size) < neg_keep)
+            X, y = X[:, keep], y[keep]
+        Xs.append(X)
+        ys.append(y)
+    sim.capture(False)
+    return np.concatenate(Xs, axis=1), np.concatenate(ys), k // every
+
+
+def choose_threshold(logits, y, recall=0.999):
+    """Largest threshold whose gate (logit > threshold) opens for at least `recall` of the positives."""
+    pos = np.sort(np.asarray(logits)[np.asarray(y) > 0.5])
+    if pos.size == 0:
+        return float("inf")
+    k = int(np.floor((1.0 - recall) * pos.size))
+    return float(np.nextafter(np.float32(pos[k]), np.float32(-np.inf)))
+
+
+def train(X, y, epochs=4, batch=1024, lr=2e-3, pos_weight=4.0, recall=0.999, holdout=0.1, seed=5, init=None):
+    """Trains on the GPU and chooses the threshold on a held-out share of the samples.
+    Returns (flat weights, threshold, info)."""
+    n = y.size
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(n)
+    nv = max(1, int(holdout * n))
+    va, tr = perm[:nv], perm[nv:]
+    Xt, yt = np.ascontiguousarray(X[:, tr]), np.ascontiguousarray(y[tr])
+    t = MlpTrainer(init_weights(seed) if init is None else init, capacity=max(tr.size, nv))
+    try:
+        t.set_data(Xt, yt)
+        losses = t.train(epochs, batch=batch, lr=lr, pos_weight=pos_weight, seed=seed)
+        w = t.weights()
+        z = t.forward(np.ascontiguousarray(X[:, va]))
+    finally:
+        t.close()
+    yv = y[va]
+    thr = choose_threshold(z, yv, recall)
+    gate = z > thr
+    info = {"samples": int(n), "positives": float(y.mean()), "epoch_loss": [float(v) for v in losses],
+            "val_recall": float(gate[yv > 0.5].mean()) if (yv > 0.5).any() else 1.0,
+            "val_open": float(gate.mean()), "threshold": thr}
+    return w, thr, info

@@ -273,6 +273,83 @@ rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char* bcis_path
 /* Frees a mesh from rkdg_mesh_generate or rkdg_mesh_read_adcirc. */
 void rkdg_mesh_free(rkdg_mesh* mesh);
 
+/* ---- learned troubled-cell indicator ------------------------------------------------
+ * No counterpart in the reference.  A 36 -> 64 -> 64 -> 64 -> 1 ReLU MLP predicts, per element,
+ * whether the Cockburn-Shu limiter (slope_limiting_kernel, swe_CS_slope_limiter.hpp:88-189) would
+ * write back a change that matters: larger than 1e-8 of the element's physical scale (FP64 forms)
+ * or 1e-6 (FP32 bulk, that form's own rounding level per step); with an indicator set, the
+ * limiter's trouble check and dilation passes are skipped and its limit pass runs only where the
+ * network's logit exceeds `threshold` (elements where it runs are limited exactly as by the plain
+ * limiter).  Inputs: modes 0-2 of ze/qx/qy of the element and its three neighbours (features as
+ * in dgswemv2_amd/csrc/indicator.hpp).  Inference and training run as BF16 MFMA products with
+ * FP32 accumulation. */
+#define RKDG_MLP_IN 36
+#define RKDG_MLP_HIDDEN 64
+typedef struct rkdg_mlp_weights {
+    float w1[RKDG_MLP_HIDDEN * RKDG_MLP_IN]; /* row-major [out][in] */
+    float b1[RKDG_MLP_HIDDEN];
+    float w2[RKDG_MLP_HIDDEN * RKDG_MLP_HIDDEN];
+    float b2[RKDG_MLP_HIDDEN];
+    float w3[RKDG_MLP_HIDDEN * RKDG_MLP_HIDDEN];
+    float b3[RKDG_MLP_HIDDEN];
+    float w4[RKDG_MLP_HIDDEN];
+    float b4;
+    float threshold; /* gate opens where logit > threshold */
+} rkdg_mlp_weights;
+
+/* Counters since the indicator was set (or since the last read with reset): element-stages. */
+typedef struct rkdg_indicator_stats {
+    int64_t stages;             /* limiter stages run with the indicator */
+    int64_t candidates;         /* element and neighbours wet (the limiter may act) */
+    int64_t flagged;            /* gate open */
+    /* with judging on (the plain limiter's decision evaluated on the same state): */
+    int64_t write_backs;        /* the plain limiter writes back */
+    int64_t significant;        /* ... a change that matters (above 1e-8 / 1e-6 of the physical scale) */
+    int64_t missed;             /* write-backs with the gate closed */
+    int64_t missed_significant; /* significant write-backs with the gate closed */
+    double max_missed_change;   /* largest relative change among missed write-backs */
+} rkdg_indicator_stats;
+
+/* Sets (w != NULL) or removes (w == NULL: the plain limiter again) the indicator of a single-mesh
+ * context with the slope limiter on.  judge != 0 also runs the plain limiter's decision as a dry
+ * run each stage and counts misses (slower; for validation). */
+rkdg_status rkdg_set_indicator(rkdg_ctx* ctx, const rkdg_mlp_weights* w, int32_t judge);
+rkdg_status rkdg_get_indicator_stats(rkdg_ctx* ctx, rkdg_indicator_stats* stats, int32_t reset);
+/* With capture on, every limiter stage records, for its state before limiting, the features
+ * ([36][n_elements]), the label (1 where the plain limiter writes back a change that matters, as
+ * above) and the size of that change (relative to the physical scale, 0 where the limiter does not
+ * write back), whether the limiter may act (element and neighbours wet), and with an indicator
+ * set the logits.  rkdg_get_capture reads the last stage's record (NULL pointers are skipped). */
+rkdg_status rkdg_capture(rkdg_ctx* ctx, int32_t on);
+rkdg_status rkdg_get_capture(rkdg_ctx* ctx, float* features, uint8_t* label, uint8_t* may_act, float* logit,
+                             float* change);
+
+/* Trainer: the same network trained on the GPU (weighted binary cross-entropy on the logit,
+ * mean over the batch, weight pos_weight on positive samples; Adam on FP32 master weights). */
+typedef struct rkdg_mlp_trainer rkdg_mlp_trainer;
+typedef struct rkdg_train_params {
+    int32_t batch;     /* samples per step (any size >= 1) */
+    int32_t epochs;
+    float lr, beta1, beta2, eps;
+    float pos_weight;
+    uint64_t seed;     /* sample order */
+} rkdg_train_params;
+/* capacity: the largest sample count rkdg_mlp_set_data / forward / gradient will be given. */
+rkdg_status rkdg_mlp_trainer_create(const rkdg_mlp_weights* init, int32_t capacity, rkdg_mlp_trainer** out);
+/* Training set: x [36][n] (feature-major), y [n] in {0, 1}. */
+rkdg_status rkdg_mlp_set_data(rkdg_mlp_trainer* t, const float* x, const float* y, int32_t n);
+/* Runs epochs over the training set; epoch_loss[epochs] (may be NULL): mean batch loss per epoch. */
+rkdg_status rkdg_mlp_train(rkdg_mlp_trainer* t, const rkdg_train_params* prm, float* epoch_loss);
+/* Current weights (threshold as given at create). */
+rkdg_status rkdg_mlp_get_weights(rkdg_mlp_trainer* t, rkdg_mlp_weights* out);
+/* Logits of x [36][n] under the current weights (the trainer's forward kernel). */
+rkdg_status rkdg_mlp_forward(rkdg_mlp_trainer* t, const float* x, int32_t n, float* logit);
+/* Loss and gradient (weights layout; threshold 0) of one batch x [36][n], y [n], n <= capacity. */
+rkdg_status rkdg_mlp_gradient(rkdg_mlp_trainer* t, const float* x, const float* y, int32_t n, float pos_weight,
+                              float* loss, rkdg_mlp_weights* grad);
+const char* rkdg_mlp_trainer_error(const rkdg_mlp_trainer* t);
+void rkdg_mlp_trainer_destroy(rkdg_mlp_trainer* t);
+
 #ifdef __cplusplus
 }
 #endif

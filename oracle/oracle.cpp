@@ -592,6 +592,8 @@ struct Sim {
     std::vector<double> frequency, forcing_fact, equilib_arg;
     double h_o = 0.1, h_o_threshold = 1.0e-6, M = 1.0e-8, nu = 1.5;
     int nthreads = 1;
+    bool keep_prelimit = false;       // keep each stage's state as it was before the limiter
+    std::vector<double> prelimit;     // [3][ndof][ne]
     std::vector<std::vector<unsigned>> intface_colours;  // for the OpenMP path
     int nan_elt = -1, nan_step = -1, nan_stage = -1;
     unsigned ndof() const { return master.ndof; }
@@ -1540,6 +1542,14 @@ static int stage(Sim& S) {
 #pragma omp parallel for num_threads(S.nthreads) schedule(static) if (S.nthreads > 1)
         for (int e = 0; e < ne; ++e) wetting_drying_kernel(S, S.elts[e]);
     }
+    if (S.keep_prelimit) {
+        const size_t nd = S.master.ndof;
+        S.prelimit.resize(3 * nd * ne);
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < 3; ++k)
+                for (size_t d = 0; d < nd; ++d)
+                    S.prelimit[(k * nd + d) * ne + e] = S.elts[e].q[S.stepper.stage](k, (int)d);
+    }
     if (S.prm.slope_limiting) slope_limiter(S);
     for (auto& E : S.elts)
         if (scrutinize(S, E)) {
@@ -1588,6 +1598,26 @@ void* orc_create(const rkdg_mesh_desc* md, const rkdg_params* prm) { return orc:
 const char* orc_error() { return orc::g_err; }
 void orc_destroy(void* h) { delete (orc::Sim*)h; }
 void orc_set_threads(void* h, int n) { ((orc::Sim*)h)->nthreads = n; }
+void orc_keep_prelimit(void* h, int on) { ((orc::Sim*)h)->keep_prelimit = on != 0; }
+// Neighbour element across each local face ([3][ne], -1 on a boundary) and the bathymetry at
+// each barycentre ([ne], only with the limiter): the inputs of the learned indicator's features.
+void orc_limiter_geometry(void* h, int* neighbor, double* bath_bc) {
+    orc::Sim& S = *(orc::Sim*)h;
+    const size_t ne = S.elts.size();
+    for (size_t e = 0; e < ne; ++e) {
+        const orc::Element& E = S.elts[e];
+        for (int k = 0; k < 3; ++k)
+            neighbor[k * ne + e] = E.neighbor_ID[k] == orc::DEFAULT_ID ? -1 : (int)E.neighbor_ID[k];
+        if (bath_bc) bath_bc[e] = S.prm.slope_limiting ? E.bath_at_baryctr : 0.0;
+    }
+}
+// State of the last stage before the limiter ran; 0 if none was kept.
+int orc_get_prelimit(void* h, double* q) {
+    orc::Sim& S = *(orc::Sim*)h;
+    if (S.prelimit.empty()) return 0;
+    std::copy(S.prelimit.begin(), S.prelimit.end(), q);
+    return 1;
+}
 int orc_max_threads() {
 #ifdef _OPENMP
     return omp_get_max_threads();

@@ -27,6 +27,9 @@ def load():
             "orc_error": (C.c_char_p, []),
             "orc_destroy": (None, [C.c_void_p]),
             "orc_set_threads": (None, [C.c_void_p, C.c_int]),
+            "orc_keep_prelimit": (None, [C.c_void_p, C.c_int]),
+            "orc_get_prelimit": (C.c_int, [C.c_void_p, dp]),
+            "orc_limiter_geometry": (None, [C.c_void_p, ip, dp]),
             "orc_max_threads": (C.c_int, []),
             "orc_stage": (C.c_int, [C.c_void_p]),
             "orc_step": (C.c_int, [C.c_void_p, C.c_int]),
@@ -97,6 +100,23 @@ class OracleSim:
         q = np.ascontiguousarray(q, np.float64).reshape(-1)
         w = None if wet is None else np.ascontiguousarray(wet, np.uint8)
         self.lib.orc_set_state(self.h, N.dptr(q), N.uptr(w))
+
+    def keep_prelimit(self, on=True):
+        """Keep each stage's state as it was after wetting-drying, before the limiter."""
+        self.lib.orc_keep_prelimit(self.h, 1 if on else 0)
+
+    def prelimit(self):
+        q = np.empty(3 * self.ndof * self.ne)
+        if not self.lib.orc_get_prelimit(self.h, N.dptr(q)):
+            raise RuntimeError("no pre-limiter state kept")
+        return q.reshape(3, self.ndof, self.ne)
+
+    def limiter_geometry(self):
+        """(neighbor [3][ne] by local face, -1 on a boundary; bathymetry at each barycentre [ne])."""
+        nb = np.empty(3 * self.ne, np.int32)
+        bb = np.empty(self.ne)
+        self.lib.orc_limiter_geometry(self.h, N.iptr(nb), N.dptr(bb))
+        return nb.reshape(3, self.ne), bb
 
     def time(self):
         t, s, k = C.c_double(), C.c_int32(), C.c_int32()
