@@ -261,10 +261,9 @@ struct Engine final : EngineBase {
     hipError_t learned_limiter(const rkdg::LimiterArgs<Real>& L) {
         rkdg::GateArgs<Real> G = gate_args(L);
         G.wpack = ind_pack, G.vec = ind_vec, G.baryctr = baryctr, G.flag = troubled;
-        G.limited = limited, G.list = ind_list, G.list_count = ind_count;
+        G.limited = limited, G.list = ind_list, G.list_count = ind_count, G.list_cap = rkdg::gate_grid(M.ne).cap;
         if (cap_on) G.logit = cap_logit;
-        hipError_t e = hipMemsetAsync(ind_count, 0, sizeof(int), stream);
-        if (e == hipSuccess) e = rkdg::launch_gate<Real>(M.p, G, true, stream);
+        hipError_t e = rkdg::launch_gate<Real>(M.p, G, true, stream);
         if (e == hipSuccess && (ind_judge || cap_on)) {
             rkdg::JudgeArgs J = judge_args();
             J.flag = troubled;
@@ -299,7 +298,8 @@ struct Engine final : EngineBase {
         std::copy(w->w4, w->w4 + 64, vec.begin() + 192);
         vec[256] = w->b4, vec[257] = w->threshold;
         if (!ind_pack && (!alloc(ind_pack, pack.size()) || !alloc(ind_vec, vec.size()) || !alloc(ind_cnt, 8) ||
-                          !alloc(ind_list, (size_t)M.ne) || !alloc(ind_count, 1)))
+                          !alloc(ind_list, (size_t)rkdg::gate_grid(M.ne).waves * rkdg::gate_grid(M.ne).cap) ||
+                          !alloc(ind_count, (size_t)rkdg::gate_grid(M.ne).waves)))
             return RKDG_ERR_HIP;
         hipError_t e = hipStreamSynchronize(stream);
         if (e == hipSuccess) e = hipMemcpy(ind_pack, pack.data(), pack.size() * sizeof(uint16_t), hipMemcpyHostToDevice);

@@ -25,15 +25,19 @@ __device__ __forceinline__ f32x4 mfma(const bf16x8& a, const bf16x8& b, const f3
 
 __device__ __forceinline__ float bf16_to_float(__bf16 b) { return (float)b; }
 
-// One hidden layer of 16 elements: D = W B + bias, W packed [mt][ks] (A operands), B in two
-// K-steps.  acc[mt][jj] = neuron 16 mt + 4 g + jj of element n.
+// max(x, 0) in one instruction (v_med3_f32); fmaxf would also canonicalise NaNs
+__device__ __forceinline__ float relu(float x) { return __builtin_amdgcn_fmed3f(x, 0.0f, 3.40282347e38f); }
+
+// One hidden layer of 16 elements: D = W B (+ bias), W packed [mt][ks] (A operands), B in two
+// K-steps.  acc[mt][jj] = neuron 16 mt + 4 g + jj of element n; the bias of those four neurons
+// is one 16-byte read.
+template <bool BIAS>
 __device__ __forceinline__ void layer(const bf16x8 (&W)[4][2], const bf16x8& b0, const bf16x8& b1, const float* bias,
                                       int g, f32x4 (&acc)[4]) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
-        f32x4 c;
-#pragma unroll
-        for (int jj = 0; jj < 4; ++jj) c[jj] = bias ? bias[16 * mt + 4 * g + jj] : 0.0f;
+        f32x4 c = {0.0f, 0.0f, 0.0f, 0.0f};
+        if constexpr (BIAS) c = *reinterpret_cast<const f32x4*>(bias + 16 * mt + 4 * g);
         c = mfma(W[mt][0], b0, c);
         acc[mt] = mfma(W[mt][1], b1, c);
     }
@@ -43,10 +47,10 @@ __device__ __forceinline__ void layer(const bf16x8 (&W)[4][2], const bf16x8& b0,
 __device__ __forceinline__ void relu_pack(const f32x4 (&acc)[4], bf16x8& b0, bf16x8& b1) {
 #pragma unroll
     for (int jj = 0; jj < 4; ++jj) {
-        b0[jj] = (__bf16)fmaxf(acc[0][jj], 0.0f);
-        b0[4 + jj] = (__bf16)fmaxf(acc[1][jj], 0.0f);
-        b1[jj] = (__bf16)fmaxf(acc[2][jj], 0.0f);
-        b1[4 + jj] = (__bf16)fmaxf(acc[3][jj], 0.0f);
+        b0[jj] = (__bf16)relu(acc[0][jj]);
+        b0[4 + jj] = (__bf16)relu(acc[1][jj]);
+        b1[jj] = (__bf16)relu(acc[2][jj]);
+        b1[4 + jj] = (__bf16)relu(acc[3][jj]);
     }
 }
 
@@ -63,6 +67,13 @@ __device__ __forceinline__ void mask_pack(const f32x4 (&acc)[4], const bf16x8& h
 }
 
 __device__ __forceinline__ uint16_t bits(__bf16 b) { return __builtin_bit_cast(uint16_t, b); }
+
+// Orders a wave's LDS writes before its later LDS reads of other lanes' data (no work-group barrier).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // store a packed pair (neurons of M-tiles 2 ks, 2 ks + 1) as rows of a [64][ld] bf16 array
 __device__ __forceinline__ void store_rows(uint16_t* dst, long long ld, int col, int g, const bf16x8& b0,
@@ -89,15 +100,58 @@ __device__ __forceinline__ void store_rows(uint16_t* dst, long long ld, int col,
 // lane per element: the gate flag.
 constexpr int kGateRow = 4 * 16 + 8;  // bf16 per element row in LDS (padded: 144 B)
 
+// Phase-1 inputs of one element: the modes 0..2 of the element and its three neighbours, their
+// wet flags, the bathymetry at the barycentre.
+template <typename Real>
+struct GateIn {
+    Real val[36];
+    uint8_t wet[4];
+    double bath;
+};
+
+template <typename Real, int P>
+__device__ __forceinline__ void gate_load(const GateArgs<Real>& G, int e, GateIn<Real>& in) {
+    constexpr int ND = Sizes<P>::ND;
+    const int ne = G.ne;
+    const int ec = e < ne ? e : ne - 1;
+    int E[4];
+    E[0] = ec;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int nb = ldc(G.neighbor, ne, k, ec);
+        E[k + 1] = nb >= 0 ? nb : ec;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+#pragma unroll
+        for (int v = 0; v < 3; ++v)
+#pragma unroll
+            for (int i = 0; i < 3; ++i) in.val[9 * k + 3 * v + i] = ldc(G.q, ne, v * ND + i, E[k]);
+        in.wet[k] = G.wet[E[k]];
+    }
+    in.bath = ldc(G.bath_bc, ne, 0, ec);
+}
+
+// Gate.  Each wave walks 64-element tiles on its own (XCD x's waves take the contiguous range
+// [x T/8, (x+1) T/8) so neighbour gathers stay in its L2; a wave only reads LDS rows it wrote, so
+// there is no work-group barrier).  Phase 1, one lane per element: the 36 features, the may-act
+// flag, the barycentres; the features go to LDS as bf16 rows of four 16-slot entity blocks.
+// Phase 2, per 16 elements: lane (g, n) reads entity g's slots of element n as its two B operands
+// and the network runs as 24 MFMA products with the weights resident in VGPRs; the logit is
+// reduced over the four lanes of each element.  Phase 3, one lane per element: the gate flag and
+// the list.  With FP32 state the next tile's inputs are loaded before phase 2, so their latency
+// hides behind the products (two dependent gathers per tile otherwise go uncovered).
 template <typename Real, int P, bool MLP>
 __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles) {
-    constexpr int ND = Sizes<P>::ND;
-    __shared__ float s_vec[mlp::NVEC];
+    constexpr bool kPrefetch = MLP && sizeof(Real) == 4;
+    __shared__ __attribute__((aligned(16))) float s_vec[mlp::NVEC];
     __shared__ __attribute__((aligned(16))) uint16_t s_x[MLP ? 256 * kGateRow : 1];
     __shared__ float s_z[MLP ? 256 : 1];
     __shared__ uint8_t s_ok[MLP ? 256 : 1];
-    if constexpr (MLP)
+    if constexpr (MLP) {
         for (int i = threadIdx.x; i < mlp::NVEC; i += blockDim.x) s_vec[i] = G.vec[i];
+        __syncthreads();
+    }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     bf16x8 W[3][4][2];
     if constexpr (MLP) {
@@ -109,24 +163,20 @@ __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles)
 #pragma unroll
                 for (int ks = 0; ks < 2; ++ks) W[L][mt][ks] = pw[((L * 4 + mt) * 2 + ks) * 64 + lane];
     }
-    const int xcd = blockIdx.x & 7, per = gridDim.x >> 3;
+    const int wpb = blockDim.x >> 6;
+    const int xcd = blockIdx.x & 7, per = (gridDim.x >> 3) * wpb;
     const int t0 = (int)((long long)ntiles * xcd / 8), t1 = (int)((long long)ntiles * (xcd + 1) / 8);
     const int ne = G.ne;
-    for (int tile = t0 + (int)(blockIdx.x >> 3); tile < t1; tile += per) {
-        // ---- phase 1: one element per lane (column loads: the column offset is scalar)
-        const int e = tile * 256 + (int)threadIdx.x;
+    int tile = t0 + (int)(blockIdx.x >> 3) * wpb + wave;
+    const int wid = blockIdx.x * wpb + wave;  // this wave's list segment
+    int nopen = 0;
+    GateIn<Real> in;
+    if (kPrefetch && tile < t1) gate_load<Real, P>(G, tile * 64 + lane, in);
+    for (; tile < t1; tile += per) {
+        // ---- phase 1: one element per lane
+        const int e = tile * 64 + lane;
         const bool valid = e < ne;
-        const int ec = valid ? e : ne - 1;
-        int E[4];
-        E[0] = ec;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int nb = ldc(G.neighbor, ne, k, ec);
-            E[k + 1] = nb >= 0 ? nb : ec;
-        }
-        Real self0[3];
-#pragma unroll
-        for (int v = 0; v < 3; ++v) self0[v] = ldc(G.q, ne, v * ND, ec);
+        if (!kPrefetch) gate_load<Real, P>(G, e, in);
         float x[36];
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -134,12 +184,12 @@ __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles)
             for (int v = 0; v < 3; ++v)
 #pragma unroll
                 for (int i = 0; i < 3; ++i) {
-                    const Real val = ldc(G.q, ne, v * ND + i, E[k]);
-                    x[9 * k + 3 * v + i] = (float)(i == 0 ? val - self0[v] : val);
+                    const Real val = in.val[9 * k + 3 * v + i];
+                    x[9 * k + 3 * v + i] = (float)(i == 0 ? val - in.val[3 * v] : val);
                 }
-        const bool okw = (G.wet[E[0]] & G.wet[E[1]] & G.wet[E[2]] & G.wet[E[3]]) != 0;
+        const bool okw = (in.wet[0] & in.wet[1] & in.wet[2] & in.wet[3]) != 0;
         // physical scales; a variation at or below 1e-16 of its scale carries no shape
-        float h = (float)((double)self0[0] + ldc(G.bath_bc, ne, 0, ec));
+        float h = (float)((double)in.val[0] + in.bath);
         h = h > (float)mlp::H_FLOOR ? h : (float)mlp::H_FLOOR;
         const float sq = h * sqrtf((float)G.g * h);
 #pragma unroll
@@ -164,11 +214,10 @@ __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles)
 #pragma unroll
                 for (int f = 0; f < 36; ++f) stc(G.feat, ne, f, e, x[f]);
             if (G.ok) G.ok[e] = okw ? 1 : 0;
-            if (G.baryctr && G.wet[e])  // as sl_prepare_element
+            if (G.baryctr && in.wet[0])  // as sl_prepare_element
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                    const double m[3] = {(double)ldc(G.q, ne, k * ND, e), (double)ldc(G.q, ne, k * ND + 1, e),
-                                         (double)ldc(G.q, ne, k * ND + 2, e)};
+                    const double m[3] = {(double)in.val[3 * k], (double)in.val[3 * k + 1], (double)in.val[3 * k + 2]};
                     double ql[3];
                     to_vertices(m, ql);
                     stc(G.baryctr, ne, k, e, (ql[0] * (1.0 / 3.0) + ql[1] * (1.0 / 3.0)) + ql[2] * (1.0 / 3.0));
@@ -189,7 +238,8 @@ __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles)
                 *reinterpret_cast<bf16x8*>(row + 16 * g + 8) = b;
             }
             s_ok[threadIdx.x] = okw ? 1 : 0;
-            __syncthreads();
+            wave_sync();
+            if (kPrefetch && tile + per < t1) gate_load<Real, P>(G, (tile + per) * 64 + lane, in);
             // ---- phase 2: the wave's 64 elements as four 16-element products
             const int g = lane >> 4, n = lane & 15;
 #pragma unroll 1
@@ -199,24 +249,24 @@ __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles)
                 bf16x8 b0 = *reinterpret_cast<const bf16x8*>(rr);
                 bf16x8 b1 = *reinterpret_cast<const bf16x8*>(rr + 8);
                 f32x4 acc[4];
-                layer(W[0], b0, b1, s_vec, g, acc);
+                layer<true>(W[0], b0, b1, s_vec, g, acc);
                 relu_pack(acc, b0, b1);
-                layer(W[1], b0, b1, s_vec + 64, g, acc);
+                layer<true>(W[1], b0, b1, s_vec + 64, g, acc);
                 relu_pack(acc, b0, b1);
-                layer(W[2], b0, b1, s_vec + 128, g, acc);
+                layer<true>(W[2], b0, b1, s_vec + 128, g, acc);
                 float part = 0.0f;
 #pragma unroll
-                for (int mt = 0; mt < 4; ++mt)
+                for (int mt = 0; mt < 4; ++mt) {
+                    const f32x4 w4 = *reinterpret_cast<const f32x4*>(s_vec + 192 + 16 * mt + 4 * g);
 #pragma unroll
-                    for (int jj = 0; jj < 4; ++jj) part += s_vec[192 + 16 * mt + 4 * g + jj] * fmaxf(acc[mt][jj], 0.0f);
+                    for (int jj = 0; jj < 4; ++jj) part = fmaf(w4[jj], relu(acc[mt][jj]), part);
+                }
                 part += __shfl_xor(part, 16);
                 part += __shfl_xor(part, 32);
                 if (g == 0) s_z[r] = part + s_vec[256];
             }
-            __syncthreads();
-            // ---- phase 3: one element per lane; open elements are appended to the limit list
-            // (one atomic per wave; the list order does not matter: each element's limit reads
-            // only its own modes and the barycentres written above)
+            wave_sync();
+            // ---- phase 3: one element per lane; open elements go to this wave's list segment
             const float z = s_z[threadIdx.x];
             const bool open = valid && s_ok[threadIdx.x] && z > s_vec[257];
             if (valid) {
@@ -226,23 +276,20 @@ __global__ void __launch_bounds__(256) gate_kernel(GateArgs<Real> G, int ntiles)
                 for (int k = 0; k < 3; ++k) G.limited[(size_t)k * ne + e] = 0;
             }
             const unsigned long long bal = __ballot(open);
-            if (bal) {
-                int base = 0;
-                if (lane == 0) base = atomicAdd(G.list_count, __popcll(bal));
-                base = __shfl(base, 0);
-                if (open) G.list[base + __popcll(bal & ((1ull << lane) - 1ull))] = e;
-            }
+            if (open) G.list[(size_t)wid * G.list_cap + nopen + __popcll(bal & ((1ull << lane) - 1ull))] = e;
+            nopen += __popcll(bal);
         }
     }
+    if (MLP && lane == 0) G.list_count[wid] = nopen;
 }
 
 template <typename Real>
 hipError_t launch_gate(int p, const GateArgs<Real>& G, bool mlp_on, hipStream_t s) {
     if (G.ne <= 0) return hipSuccess;
     if (mlp_on && (!G.wpack || !G.vec || !G.flag || !G.limited || !G.list || !G.list_count)) return hipErrorInvalidValue;
-    const int ntiles = (G.ne + 255) / 256;  // 256-element tiles, one per work-group iteration
-    int per = (ntiles + 7) / 8;             // work-groups per XCD: at most two per CU (32 CUs per XCD)
-    per = per < 64 ? per : 64;
+    const GateGrid gg = gate_grid(G.ne);
+    if (mlp_on && G.list_cap != gg.cap) return hipErrorInvalidValue;
+    const int ntiles = gg.ntiles, per = gg.per_xcd;
     const dim3 grid(8 * per), block(256);
     if (p == 1) {
         if (mlp_on)
@@ -270,7 +317,7 @@ template hipError_t launch_gate<double>(int, const GateArgs<double>&, bool, hipS
 template <bool BACKWARD>
 __global__ void __launch_bounds__(256) mlp_fwdbwd_kernel(TrainArgs T) {
     __shared__ bf16x8 s_w[mlp::NPACKED * mlp::PACK / 8];
-    __shared__ float s_vec[4 * mlp::NH + 1];
+    __shared__ __attribute__((aligned(16))) float s_vec[4 * mlp::NH + 1];
     {
         const bf16x8* pw = reinterpret_cast<const bf16x8*>(T.wpack);
         const int nw = (BACKWARD ? mlp::NPACKED : 3) * mlp::PACK / 8;
@@ -312,13 +359,13 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd_kernel(TrainArgs T) {
     f32x4 acc[4];
     bf16x8 h1a, h1b, h2a, h2b;
     wmat(0, W);
-    layer(W, b0, b1, s_vec, g, acc);
+    layer<true>(W, b0, b1, s_vec, g, acc);
     relu_pack(acc, h1a, h1b);
     wmat(1, W);
-    layer(W, h1a, h1b, s_vec + 64, g, acc);
+    layer<true>(W, h1a, h1b, s_vec + 64, g, acc);
     relu_pack(acc, h2a, h2b);
     wmat(2, W);
-    layer(W, h2a, h2b, s_vec + 128, g, acc);
+    layer<true>(W, h2a, h2b, s_vec + 128, g, acc);
     float h3[4][4];
     float part = 0.0f;
 #pragma unroll
@@ -366,12 +413,12 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd_kernel(TrainArgs T) {
     store_rows(T.dz3, T.Bpad, p, g, d0, d1);
     // dZ2 = (W3^T dZ3) (H2 > 0)
     wmat(4, W);
-    layer(W, d0, d1, nullptr, g, acc);
+    layer<false>(W, d0, d1, nullptr, g, acc);
     mask_pack(acc, h2a, h2b, d0, d1);
     store_rows(T.dz2, T.Bpad, p, g, d0, d1);
     // dZ1 = (W2^T dZ2) (H1 > 0)
     wmat(3, W);
-    layer(W, d0, d1, nullptr, g, acc);
+    layer<false>(W, d0, d1, nullptr, g, acc);
     mask_pack(acc, h1a, h1b, d0, d1);
     store_rows(T.dz1, T.Bpad, p, g, d0, d1);
 }

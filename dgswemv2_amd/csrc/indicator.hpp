@@ -106,9 +106,33 @@ struct GateArgs {
     float* feat;              // [36][ne] or null
     uint8_t* ok;              // [ne] element and neighbours wet, or null
     uint8_t* limited;         // [3][ne] cleared (the limit pass sets it where it writes), MLP only
-    int* list;                // [ne] elements whose gate is open, in no particular order, MLP only
-    int* list_count;          // their number (zero on entry), MLP only
+    int* list;                // elements whose gate is open: wave w's in list[w cap ..], MLP only
+    int* list_count;          // [waves] how many each wave found, MLP only
+    int list_cap;             // gate_grid(ne).cap
 };
+
+// Launch shape of the gate (and of the limit pass over its list): work-groups of 4 waves, each
+// wave walking 64-element tiles of its XCD's contiguous range; `cap` bounds the tiles (x 64
+// elements) one wave can meet, so wave w's list segment is [w cap, (w + 1) cap).
+struct GateGrid {
+    int ntiles, per_xcd, waves, cap;
+};
+inline GateGrid gate_grid(int ne) {
+    GateGrid g;
+    g.ntiles = (ne + 63) / 64;
+    g.per_xcd = (g.ntiles + 31) / 32;          // work-groups per XCD: at most two per CU (32 CUs)
+    g.per_xcd = g.per_xcd < 64 ? g.per_xcd : 64;
+    g.per_xcd = g.per_xcd > 0 ? g.per_xcd : 1;
+    g.waves = 8 * g.per_xcd * 4;
+    int most = 0;                              // the largest XCD range, in tiles
+    for (int x = 0; x < 8; ++x) {
+        const int t0 = (int)((long long)g.ntiles * x / 8), t1 = (int)((long long)g.ntiles * (x + 1) / 8);
+        most = t1 - t0 > most ? t1 - t0 : most;
+    }
+    const int stride = g.per_xcd * 4;          // waves per XCD
+    g.cap = 64 * ((most + stride - 1) / stride);
+    return g;
+}
 
 // Dry-run limiter over all elements: labels and the judge's counters.
 struct JudgeArgs {

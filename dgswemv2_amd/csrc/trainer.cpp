@@ -43,8 +43,8 @@ struct rkdg_mlp_trainer {
     long long t = 0;             // Adam steps taken
 
     ~rkdg_mlp_trainer() {
-        for (void* p : allocs) hipFree(p);
-        if (stream) hipStreamDestroy(stream);
+        for (void* p : allocs) (void)hipFree(p);
+        if (stream) (void)hipStreamDestroy(stream);
     }
     bool fail(const std::string& what, hipError_t e) {
         err = what + ": " + hipGetErrorString(e);
@@ -61,7 +61,7 @@ struct rkdg_mlp_trainer {
     }
     bool release(void* p) {
         auto it = std::find(allocs.begin(), allocs.end(), p);
-        if (it != allocs.end()) hipFree(*it), allocs.erase(it);
+        if (it != allocs.end()) (void)hipFree(*it), allocs.erase(it);
         return true;
     }
     static int pad(int b) { return (b + 31) / 32 * 32; }
