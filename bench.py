@@ -12,6 +12,8 @@ value = DOF * stages / s of the headline form: the fastest form within its gate.
 stage-kernel time from HIP events on the context's stream, against 8 TB/s.
 cpu_baseline: the FP64 CPU oracle (OpenMP over host cores) on a bounded sample of the same
 recipe, timed on this host.  ensemble: the 64-instance ensemble (FP32 bulk, autotuned schedule).
+learned: the 500k-triangle dam break in FP64 plain, FP32 plain and FP32 with the learned
+troubled-cell gate (trained on the GPU on the plain limiter's decisions during a warm-up).
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -116,6 +118,74 @@ def ensemble_bench(steps, warmup, scale):
             "dtype": "f32"}
 
 
+def learned_bench(steps, scale, warm=40, judge_steps=10):
+    """Learned troubled-cell indicator on the 500k-triangle dam break (p = 1, SSP-RK2, wetting-drying,
+    Cockburn-Shu limiter).  FP32 bulk with the plain limiter runs `warm` steps with capture on
+    (features and labels of every limiter stage), the MLP is trained on the GPU, and then `steps`
+    steps are timed from the warm state in FP64 plain, FP32 plain and FP32 with the learned gate.
+    A further `judge_steps` window with the judge on (the plain limiter's decision on the learned
+    run's states) gives the miss rate; L2 errors are taken against FP64 plain after the same steps."""
+    import numpy as np
+    from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_PLAIN, Simulation, indicator as IND, workloads as W
+
+    def rel(a, b):
+        return [float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300)) for k in range(3)]
+
+    log("building the dam-break mesh, scale", scale)
+    mesh, prm = W.dambreak(scale=scale)
+    ne, ndof, ns = mesh.n_elements, 3, prm.nstages
+    a = Simulation(mesh, prm, RKDG_FP32_BULK)  # plain limiter: labels the warm-up steps
+    t0 = time.perf_counter()
+    X, y, nst = IND.collect(a, warm, every=1, neg_keep=0.25, seed=1)
+    collect_s = time.perf_counter() - t0
+    log("collected", y.size, "samples from", nst, "limiter stages in", round(collect_s, 2), "s")
+    t0 = time.perf_counter()
+    w, thr, info = IND.train(X, y, epochs=6, batch=1024, pos_weight=8.0, recall=0.9999, margin=6.0)
+    train_s = time.perf_counter() - t0
+    del X, y
+    log("trained in", round(train_s, 2), "s:", json.dumps(info))
+    b = Simulation(mesh, prm, RKDG_FP32_BULK)
+    b.step(warm)
+    b.set_indicator(w, threshold=thr)
+    r = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    r.step(warm)
+    res, qs = {}, {}
+    for name, sim in (("fp64_plain", r), ("fp32_plain", a), ("fp32_learned", b)):
+        sim.sync()
+        sim.reset_timers()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        wall = time.perf_counter() - t0
+        rep = sim.report()
+        st = max(1, rep["stages_timed"])
+        qs[name] = sim.state()["q"]
+        res[name] = {"value": dof_stages(ne, ndof, steps * ns) / wall, "ms_per_step": wall / steps * 1e3,
+                     "stage_kernel_ms": rep["stage_kernel_ms"] / st, "limiter_ms": rep["limiter_kernel_ms"] / st}
+        log(name, json.dumps(res[name]))
+    res["fp32_plain"]["rel_l2_vs_fp64_plain"] = rel(qs["fp32_plain"], qs["fp64_plain"])
+    res["fp32_learned"]["rel_l2_vs_fp64_plain"] = rel(qs["fp32_learned"], qs["fp64_plain"])
+    res["fp32_learned"]["rel_l2_vs_fp32_plain"] = rel(qs["fp32_learned"], qs["fp32_plain"])
+    b.set_indicator(w, threshold=thr, judge=True)
+    b.step(judge_steps)
+    js = b.indicator_stats()
+    for sim in (a, b, r):
+        sim.close()
+    return {"workload": f"dam break p=1 SSP-RK2, {ne} triangles (500x500 quads over 5 km, 10 m / 1 m step, dry "
+                        "floodplain, wetting-drying, Cockburn-Shu limiter)", "triangles": ne, "warm_steps": warm,
+            "timed_steps": steps, "dtype": "f32 (FP64 limiter arithmetic), bf16 MFMA gate", **res,
+            "learned_over_fp64_plain": res["fp32_learned"]["value"] / res["fp64_plain"]["value"],
+            "learned_over_fp32_plain": res["fp32_learned"]["value"] / res["fp32_plain"]["value"],
+            "training": {"samples": info["samples"], "positives": info["positives"], "epochs": 6,
+                         "collect_s": collect_s, "train_s": train_s, "val_recall": info["val_recall"],
+                         "val_open": info["val_open"], "final_loss": info["epoch_loss"][-1]},
+            "judge": {"steps": judge_steps, "candidates": js["candidates"], "gate_open": js["flagged"],
+                      "significant_write_backs": js["significant"],
+                      "missed_significant": js["missed_significant"],
+                      "missed_share": js["missed_significant"] / max(1, js["significant"]),
+                      "max_missed_change": js["max_missed_change"]}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +195,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-scale", type=float, default=1.0)
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--learned-scale", type=float, default=1.0)
     ap.add_argument("--form", choices=("all", "fp64_plain", "fp64_fast", "fp32_bulk", "none"), default="all",
                     help="time one form only (short profiled runs); the JSON line needs all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -169,6 +241,8 @@ def main():
     if args.form != "all":
         if not args.no_ensemble:
             log("ensemble", json.dumps(ensemble_bench(args.steps, args.warmup, args.ensemble_scale)))
+        if not args.no_learned:
+            log("learned", json.dumps(learned_bench(args.steps, args.learned_scale)))
         log("single form timed; no JSON line")
         return
     # errors against the FP64 plain form after the same steps from the same start state
@@ -197,6 +271,9 @@ def main():
     ens = None if args.no_ensemble else ensemble_bench(args.steps, args.warmup, args.ensemble_scale)
     if ens:
         log("ensemble", json.dumps(ens))
+    learned = None if args.no_learned else learned_bench(args.steps, args.learned_scale)
+    if learned:
+        log("learned", json.dumps(learned))
     cpu = None if args.no_cpu else cpu_baseline()
     if cpu:
         log("cpu baseline", json.dumps(cpu))
@@ -224,6 +301,7 @@ def main():
                      "frac": h["achieved_GBs"] / HBM_PEAK_GBS, "traffic": traffic},
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "learned": learned,
     }
     print(json.dumps(out), flush=True)
 

@@ -5,10 +5,10 @@ dgswemv2_amd/csrc/indicator.hpp.  The workflow here:
 
   1. run the simulation with the plain limiter and capture on (Simulation.capture): every
      limiter stage records the features of its state before limiting and the label (the plain
-     limiter writes back a change above 1e-8 of the element's physical scale);
+     limiter writes back a change above 1e-8 of the element's physical scale, 1e-6 in FP32);
   2. train the MLP on the GPU (MlpTrainer: BF16-MFMA forward / back-propagation, Adam);
   3. pick the gate threshold on held-out samples so that a target share of the positive
-     samples (recall) opens the gate;
+     samples (recall) opens the gate, then lower it by a margin;
   4. Simulation.set_indicator(weights, threshold): the limiter runs only where the gate opens.
 Only elements where the limiter may act (element and neighbours wet) are training samples;
 elsewhere the gate is closed by construction.
@@ -66,9 +66,13 @@ def choose_threshold(logits, y, recall=0.999):
     return float(np.nextafter(np.float32(pos[k]), np.float32(-np.inf)))
 
 
-def train(X, y, epochs=4, batch=1024, lr=2e-3, pos_weight=4.0, recall=0.999, holdout=0.1, seed=5, init=None):
-    """Trains on the GPU and chooses the threshold on a held-out share of the samples.
-    Returns (flat weights, threshold, info)."""
+def train(X, y, epochs=6, batch=1024, lr=2e-3, pos_weight=8.0, recall=0.9999, margin=6.0, holdout=0.1, seed=5,
+          init=None):
+    """Trains on the GPU and chooses the threshold on a held-out share of the samples: the logit at
+    which `recall` of the held-out positives open the gate, lowered by `margin` (logits are
+    log-odds).  The margin widens the gate by a few per cent of the open elements on the dam break
+    and keeps what it misses on states after training small (largest missed change 3e-5 of the
+    physical scale there, against 1e-3 without it).  Returns (flat weights, threshold, info)."""
     n = y.size
     rng = np.random.default_rng(seed)
     perm = rng.permutation(n)
@@ -84,7 +88,7 @@ def train(X, y, epochs=4, batch=1024, lr=2e-3, pos_weight=4.0, recall=0.999, hol
     finally:
         t.close()
     yv = y[va]
-    thr = choose_threshold(z, yv, recall)
+    thr = choose_threshold(z, yv, recall) - margin
     gate = z > thr
     info = {"samples": int(n), "positives": float(y.mean()), "epoch_loss": [float(v) for v in losses],
             "val_recall": float(gate[yv > 0.5].mean()) if (yv > 0.5).any() else 1.0,
