@@ -20,6 +20,7 @@
 #include "../../include/rkdg.h"
 #include "launch.hpp"
 #include "model.hpp"
+#include "writers.hpp"
 
 namespace {
 thread_local std::string g_create_error;
@@ -1211,6 +1212,28 @@ rkdg_status rkdg_get_capture(rkdg_ctx* ctx, float* features, uint8_t* label, uin
     if (st != RKDG_OK) ctx->err = ctx->eng->err;
     return st;
 }
+
+// Writes the current state at a step boundary (the reference writes state[0] from
+// Writer::WriteOutput, simulation/writer.hpp:121-196).
+static rkdg_status write_output(rkdg_ctx* ctx, const char* prefix, bool vtu) {
+    if (!ctx || !prefix) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: write each instance from its own context");
+    const rkdg::Model& M = ctx->model;
+    if (M.stepper.stage != 0) return set_err(ctx, RKDG_ERR_ARG, "output is written at a step boundary");
+    const size_t ne = M.ne;
+    std::vector<double> q(3 * (size_t)M.ndof * ne);
+    std::vector<uint8_t> wet(ne);
+    rkdg_status st = ctx->eng->get_state(q.data(), wet.data(), nullptr, nullptr);
+    if (st != RKDG_OK) return set_err(ctx, st, ctx->eng->err);
+    std::string err;
+    const bool ok = vtu ? rkdg::write_vtu(M, q.data(), wet.data(), (int)M.stepper.step, prefix, err)
+                        : rkdg::write_modal(M, q.data(), M.stepper.stage_time(), (int)M.stepper.step, prefix, err);
+    return ok ? RKDG_OK : set_err(ctx, RKDG_ERR_ARG, err);
+}
+
+rkdg_status rkdg_write_modal(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, false); }
+
+rkdg_status rkdg_write_vtu(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, true); }
 
 rkdg_status rkdg_master_tables(int32_t p, int32_t* ndof, int32_t* ngp, double* phi, double* dz1, double* dz2,
                                double* w) {

@@ -385,7 +385,7 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd_kernel(TrainArgs T) {
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-        for (int jj = 0; jj < 4; ++jj) T.h3[(long long)(16 * mt + 4 * g + jj) * T.Bpad + p] = h3[mt][jj];
+        for (int jj = 0; jj < 4; ++jj) T.h3[(long long)p * mlp::NH + 16 * mt + 4 * g + jj] = h3[mt][jj];
     // weighted BCE on the logit, mean over the batch
     const float yv = valid ? T.y[sidx] : 0.0f;
     const float w = valid ? (yv > 0.5f ? T.pos_weight : 1.0f) : 0.0f;
@@ -482,7 +482,7 @@ __global__ void __launch_bounds__(256) mlp_wgrad_kernel(TrainArgs T, float* part
         }
     } else {
         float w4 = 0.0f;
-        for (int k = k_begin; k < k_end; ++k) w4 += T.dz[k] * T.h3[(long long)lane * T.Bpad + k];
+        for (int k = k_begin; k < k_end; ++k) w4 += T.dz[k] * T.h3[(long long)k * mlp::NH + lane];
         out[mlp::OFF_W4 + lane] = w4;
         float db = 0.0f, ls = 0.0f;
         for (int k = k_begin + lane; k < k_end; k += 64) db += T.dz[k], ls += T.loss[k];

@@ -155,7 +155,7 @@ struct TrainArgs {
     const uint16_t* wpack;    // the five packed matrices
     const float* theta;       // [NPARAM] master parameters
     uint16_t *xb, *h1, *h2, *dz1, *dz2, *dz3;  // bf16 bits: [48][Bpad], [64][Bpad] x 5
-    float* h3;                // [64][Bpad]
+    float* h3;                // [Bpad][64] (sample-major: the w4 gradient walks samples)
     float* dz;                // [Bpad] dL/dlogit
     float* loss;              // [Bpad] weighted per-sample loss
     float* logit;             // [Bpad] or null

@@ -273,6 +273,17 @@ rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char* bcis_path
 /* Frees a mesh from rkdg_mesh_generate or rkdg_mesh_read_adcirc. */
 void rkdg_mesh_free(rkdg_mesh* mesh);
 
+/* ---- output ----------------------------------------------------------------------------
+ * The reference's writers, at a step boundary, single-mesh contexts.
+ * rkdg_write_modal: write_modal_data (problem_postprocessor/swe_post_write_modal.hpp:5-90):
+ *   <prefix>_modal_{ze,qx,qy,bath}.txt, created at step 0 and appended to afterwards; each record
+ *   is the time, then "ID value" per element and mode.
+ * rkdg_write_vtu: write_VTU_data + the geometry of Writer::InitializeMeshGeometryVTU
+ *   (swe_post_write_vtu.hpp:5-103, simulation/writer.hpp:260-350): <prefix>_data_<step>.vtu.
+ * prefix = output path + mesh name, as the reference composes it. */
+rkdg_status rkdg_write_modal(rkdg_ctx* ctx, const char* prefix);
+rkdg_status rkdg_write_vtu(rkdg_ctx* ctx, const char* prefix);
+
 /* ---- learned troubled-cell indicator ------------------------------------------------
  * No counterpart in the reference.  A 36 -> 64 -> 64 -> 64 -> 1 ReLU MLP predicts, per element,
  * whether the Cockburn-Shu limiter (slope_limiting_kernel, swe_CS_slope_limiter.hpp:88-189) would
