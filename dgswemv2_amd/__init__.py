@@ -17,7 +17,7 @@ from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "read_adcirc", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
+    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
@@ -153,6 +153,16 @@ def read_adcirc(fort14, bcis=None):
         return _mesh_from_desc(desc)
     finally:
         lib.rkdg_mesh_free(handle)
+
+
+def write_adcirc(mesh, fort14, bcis=None, name="mesh"):
+    """Writes a Mesh as an ADCIRC .14 (and .bcis) through the library's writer; read_adcirc of the
+    files returns the same mesh (boundary edges as a set, in the reader's order)."""
+    lib = N.load_library()
+    st = lib.rkdg_mesh_write_adcirc(C.byref(mesh.desc), name.encode(), str(fort14).encode(),
+                                    None if bcis is None else str(bcis).encode())
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
 
 
 def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, problem=PROBLEM_NONE,

@@ -7,6 +7,7 @@
 // for tide/flow constituents and levee coefficients.
 #include <algorithm>
 #include <array>
+#include <cstdio>
 #include <fstream>
 #include <map>
 #include <memory>
@@ -300,6 +301,201 @@ extern "C" rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char
         *out = M.release();
         return RKDG_OK;
     } catch (const ReadError& err) {
+        rkdg::set_create_error(err.msg);
+    } catch (const std::bad_alloc&) {
+        rkdg::set_create_error("out of host memory");
+    }
+    return RKDG_ERR_ARG;
+}
+
+namespace {
+
+// Splits a set of edges into node strings whose consecutive pairs are exactly those edges, each
+// used once: walks start at nodes of odd degree (string ends), then what is left is closed loops
+// (all degrees even), each written with its first node repeated at the end.  Deterministic: starts
+// in order of first appearance in the edge list.
+std::vector<std::vector<long>> chain_edges(const std::vector<std::array<long, 2>>& E) {
+    std::unordered_map<long, std::vector<size_t>> adj;
+    std::vector<long> order;  // nodes by first appearance
+    for (size_t i = 0; i < E.size(); ++i)
+        for (long v : E[i]) {
+            auto& a = adj[v];
+            if (a.empty()) order.push_back(v);
+            a.push_back(i);
+        }
+    std::vector<char> used(E.size(), 0);
+    std::unordered_map<long, size_t> cursor;
+    auto walk = [&](long start) {
+        std::vector<long> s{start};
+        long cur = start;
+        for (;;) {
+            auto& a = adj[cur];
+            size_t& c = cursor[cur];
+            while (c < a.size() && used[a[c]]) ++c;
+            if (c == a.size()) break;
+            const size_t i = a[c];
+            used[i] = 1;
+            cur = E[i][0] == cur ? E[i][1] : E[i][0];
+            s.push_back(cur);
+        }
+        return s;
+    };
+    std::vector<std::vector<long>> out;
+    for (long v : order)
+        if (adj[v].size() % 2 == 1)
+            while (true) {
+                auto s = walk(v);
+                if (s.size() < 2) break;
+                out.push_back(std::move(s));
+            }
+    for (size_t i = 0; i < E.size(); ++i)
+        if (!used[i]) out.push_back(walk(E[i][0]));
+    return out;
+}
+
+struct WriteError {
+    std::string msg;
+};
+
+}  // namespace
+
+extern "C" rkdg_status rkdg_mesh_write_adcirc(const rkdg_mesh_desc* d, const char* name, const char* fort14_path,
+                                              const char* bcis_path) {
+    if (!d || !fort14_path) {
+        rkdg::set_create_error("null argument");
+        return RKDG_ERR_ARG;
+    }
+    try {
+        const long nn = d->n_nodes, ne = d->n_elements, nb = d->n_bound_edges, nl = d->n_levee_pairs;
+        const int nc = d->n_constituents;
+        if (nn <= 0 || ne <= 0 || !d->node_xyz || !d->elt_nodes) throw WriteError{"empty mesh"};
+        if (nb < 0 || (nb > 0 && (!d->bound_edge_nodes || !d->bound_edge_type))) throw WriteError{"bad boundary edges"};
+        if (nl < 0 || (nl > 0 && (!d->levee_nodes || !d->levee_param))) throw WriteError{"bad levee pairs"};
+        auto check_node = [&](long v) {
+            if (v < 0 || v >= nn) throw WriteError{"node index " + std::to_string(v) + " out of range"};
+            return v;
+        };
+        for (long i = 0; i < 3 * ne; ++i) check_node(d->elt_nodes[i]);
+        for (long i = 0; i < 2 * nb; ++i) check_node(d->bound_edge_nodes[i]);
+        for (long i = 0; i < 2 * nl; ++i) check_node(d->levee_nodes[i]);
+
+        // boundary edges by code; the writer's string kinds (AdcircFormat::write_to,
+        // adcirc_format.cpp:151-212): open boundaries (NBDV) for tide, IBTYPE 0 land, 2 flow,
+        // 77 function, 88 outflow, 4 internal barrier (NBVV front, IBCONN back)
+        std::map<int, std::vector<std::array<long, 2>>> by_type;
+        std::map<std::pair<long, long>, int> edge_type;
+        for (long i = 0; i < nb; ++i) {
+            const long a = d->bound_edge_nodes[2 * i], b = d->bound_edge_nodes[2 * i + 1];
+            const int t = d->bound_edge_type[i];
+            if (t != RKDG_BC_LAND && t != RKDG_BC_TIDE && t != RKDG_BC_FLOW && t != RKDG_BC_FUNCTION &&
+                t != RKDG_BC_LEVEE && t != 4)
+                throw WriteError{"boundary code " + std::to_string(t) + " has no ADCIRC boundary type"};
+            if (t != RKDG_BC_LEVEE) by_type[t].push_back({a, b});
+            edge_type[{std::min(a, b), std::max(a, b)}] = t;
+        }
+        auto is_levee_edge = [&](long a, long b) {
+            auto it = edge_type.find({std::min(a, b), std::max(a, b)});
+            return it != edge_type.end() && it->second == RKDG_BC_LEVEE;
+        };
+        // barrier segments: the levee pairs in order, split where the front or back nodes of
+        // consecutive pairs are not joined by a levee edge
+        std::vector<std::vector<long>> bar;  // pair indices
+        for (long i = 0; i < nl; ++i) {
+            const int32_t* L = d->levee_nodes;
+            if (i == 0 || !is_levee_edge(L[2 * i - 2], L[2 * i]) || !is_levee_edge(L[2 * i - 1], L[2 * i + 1]))
+                bar.emplace_back();
+            bar.back().push_back(i);
+        }
+        // every levee edge must be a consecutive front or back pair of a barrier segment
+        std::map<std::pair<long, long>, int> covered;
+        for (const auto& s : bar)
+            for (size_t k = 1; k < s.size(); ++k)
+                for (int side = 0; side < 2; ++side) {
+                    const long a = d->levee_nodes[2 * s[k - 1] + side], b = d->levee_nodes[2 * s[k] + side];
+                    covered[{std::min(a, b), std::max(a, b)}] = 1;
+                }
+        for (const auto& kv : edge_type)
+            if (kv.second == RKDG_BC_LEVEE && !covered.count(kv.first))
+                throw WriteError{"levee edge (" + std::to_string(kv.first.first) + ", " +
+                                 std::to_string(kv.first.second) + ") joins no two consecutive levee pairs"};
+
+        FILE* f = std::fopen(fort14_path, "w");
+        if (!f) throw WriteError{std::string("cannot open ") + fort14_path};
+        std::unique_ptr<FILE, int (*)(FILE*)> guard(f, &std::fclose);
+        std::fprintf(f, "%s\n%ld %ld\n", name && *name ? name : "mesh", ne, nn);
+        for (long i = 0; i < nn; ++i)
+            std::fprintf(f, "%ld %.17g %.17g %.17g\n", i, d->node_xyz[3 * i], d->node_xyz[3 * i + 1], d->node_xyz[3 * i + 2]);
+        for (long e = 0; e < ne; ++e)
+            std::fprintf(f, "%ld 3 %d %d %d\n", e, d->elt_nodes[3 * e], d->elt_nodes[3 * e + 1], d->elt_nodes[3 * e + 2]);
+        const auto open = chain_edges(by_type[RKDG_BC_TIDE]);
+        size_t neta = 0;
+        for (const auto& s : open) neta += s.size();
+        std::fprintf(f, "%zu = Number of open boundaries\n%zu = Total number of open boundary nodes\n", open.size(), neta);
+        for (size_t k = 0; k < open.size(); ++k) {
+            std::fprintf(f, "%zu = Number of nodes for open boundary %zu\n", open[k].size(), k + 1);
+            for (long v : open[k]) std::fprintf(f, "%ld\n", v);
+        }
+        std::vector<std::pair<int, std::vector<long>>> land;  // (IBTYPE, nodes)
+        const std::pair<int, int> kinds[] = {{RKDG_BC_LAND, 0}, {RKDG_BC_FLOW, 2}, {RKDG_BC_FUNCTION, 77}, {4, 88}};
+        for (const auto& k : kinds)
+            for (auto& s : chain_edges(by_type[k.first])) land.emplace_back(k.second, std::move(s));
+        size_t nvel = 0;
+        for (const auto& s : land) nvel += s.second.size();
+        for (const auto& s : bar) nvel += 2 * s.size();
+        std::fprintf(f, "%zu = Number of land boundaries\n%zu = Total number of land boundary nodes\n",
+                     land.size() + bar.size(), nvel);
+        size_t seg = 0;
+        for (const auto& s : land) {
+            std::fprintf(f, "%zu %d = Number of nodes for land boundary %zu\n", s.second.size(), s.first, ++seg);
+            for (long v : s.second) std::fprintf(f, "%ld\n", v);
+        }
+        for (const auto& s : bar) {
+            std::fprintf(f, "%zu 4 = Number of nodes for land boundary %zu\n", s.size(), ++seg);
+            for (long i : s)
+                std::fprintf(f, "%d %d %.17g %.17g %.17g\n", d->levee_nodes[2 * i], d->levee_nodes[2 * i + 1],
+                             d->levee_param[3 * i], d->levee_param[3 * i + 1], d->levee_param[3 * i + 2]);
+        }
+        std::fprintf(f, "0 = Number of generic boundaries\n0 = Total number of generic boundary nodes\n");
+        if (std::ferror(f)) throw WriteError{std::string("write error on ") + fort14_path};
+        guard.reset();
+
+        if (bcis_path) {
+            FILE* g = std::fopen(bcis_path, "w");
+            if (!g) throw WriteError{std::string("cannot open ") + bcis_path};
+            std::unique_ptr<FILE, int (*)(FILE*)> gguard(g, &std::fclose);
+            // tide / flow blocks (SWE::Inputs::read_bcis, swe_inputs.cpp:212-296): the nodes of
+            // that boundary kind with their amplitude and phase per constituent
+            if (nc > 0) {
+                if (!d->frequency || !d->forcing_fact || !d->equilib_arg || !d->node_amplitude || !d->node_phase)
+                    throw WriteError{"constituents without node amplitudes/phases"};
+                for (int t : {(int)RKDG_BC_TIDE, (int)RKDG_BC_FLOW}) {
+                    std::vector<long> nodes;
+                    for (const auto& e : by_type[t]) nodes.push_back(e[0]), nodes.push_back(e[1]);
+                    std::sort(nodes.begin(), nodes.end());
+                    nodes.erase(std::unique(nodes.begin(), nodes.end()), nodes.end());
+                    if (nodes.empty()) continue;
+                    std::fprintf(g, "%d %zu = %s boundary\n%d = number of constituents\n", t, nodes.size(),
+                                 t == RKDG_BC_TIDE ? "tide" : "flow", nc);
+                    for (int c = 0; c < nc; ++c) {
+                        std::fprintf(g, "%.17g %.17g %.17g = frequency, forcing factor, equilibrium argument\n",
+                                     d->frequency[c], d->forcing_fact[c], d->equilib_arg[c]);
+                        for (long v : nodes)
+                            std::fprintf(g, "%ld %.17g %.17g\n", v, d->node_amplitude[(size_t)v * nc + c],
+                                         d->node_phase[(size_t)v * nc + c]);
+                    }
+                }
+            }
+            // levee block (:298-310): front, back, barrier height, sub-/supercritical coefficients
+            if (nl > 0) {
+                std::fprintf(g, "%d %ld = levee boundary\n", (int)RKDG_BC_LEVEE, nl);
+                for (long i = 0; i < nl; ++i)
+                    std::fprintf(g, "%d %d %.17g %.17g %.17g\n", d->levee_nodes[2 * i], d->levee_nodes[2 * i + 1],
+                                 d->levee_param[3 * i], d->levee_param[3 * i + 1], d->levee_param[3 * i + 2]);
+            }
+            if (std::ferror(g)) throw WriteError{std::string("write error on ") + bcis_path};
+        }
+        return RKDG_OK;
+    } catch (const WriteError& err) {
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");

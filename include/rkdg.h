@@ -273,6 +273,20 @@ rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char* bcis_path
 /* Frees a mesh from rkdg_mesh_generate or rkdg_mesh_read_adcirc. */
 void rkdg_mesh_free(rkdg_mesh* mesh);
 
+/* Writes a mesh description as an ADCIRC mesh (.14) and, if bcis_path is not NULL, its .bcis,
+ * in the layout AdcircFormat::write_to (adcirc_format.cpp:151-212) and SWE::Inputs::read_bcis
+ * (swe_inputs.cpp:212-312) use; `name` (NULL: "mesh") is the title line.  Node and element IDs
+ * are their indices; reals are printed with 17 significant digits, so rkdg_mesh_read_adcirc
+ * returns the same nodes, elements, boundary codes, levee pairs and forcing.  Boundary edges of
+ * one code are chained into node strings: tide -> open boundaries, land / flow / function /
+ * outflow (code 4) -> IBTYPE 0 / 2 / 77 / 88, levee pairs -> IBTYPE 4 barriers (split where
+ * consecutive pairs' front or back nodes are not joined by a levee edge; every levee edge must
+ * join two consecutive pairs).  The .bcis holds one tide and one flow block (the nodes of those
+ * edges, all constituents) and the levee block.  Node Manning n has no place in these files and
+ * is not written.  Errors: RKDG_ERR_ARG with the message in rkdg_create_error(). */
+rkdg_status rkdg_mesh_write_adcirc(const rkdg_mesh_desc* mesh, const char* name, const char* fort14_path,
+                                   const char* bcis_path);
+
 /* ---- output ----------------------------------------------------------------------------
  * The reference's writers, at a step boundary, single-mesh contexts.
  * rkdg_write_modal: write_modal_data (problem_postprocessor/swe_post_write_modal.hpp:5-90):

@@ -1,0 +1,94 @@
+"""ADCIRC .14 / .bcis writer (rkdg_mesh_write_adcirc) against the reader (rkdg_mesh_read_adcirc).
+
+The writer follows AdcircFormat::write_to (preprocessor/ADCIRC_reader/adcirc_format.cpp:151-212)
+and the block layout SWE::Inputs::read_bcis parses (problem_input/swe_inputs.cpp:212-312).  A
+written mesh read back must be the mesh that was written: the reference's weir fixture (flow
+boundary with two constituents, a 19-pair barrier) exactly, array for array; generated meshes
+with every boundary code as the same set of typed boundary edges; and the oracle's RKDG steps on
+the re-read mesh bit-identical to those on the original.
+"""
+import numpy as np
+import pytest
+
+from dgswemv2_amd import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, Mesh, RkdgError, make_params, quad_mesh,
+                          read_adcirc, write_adcirc)
+from oracle.oracle_py import OracleSim
+from test_levee import WEIR
+
+
+def edge_set(m):
+    return sorted((min(a, b), max(a, b), int(t)) for (a, b), t in zip(m.bound_edges.tolist(), m.bound_types))
+
+
+def test_weir_round_trip(tmp_path):
+    m = read_adcirc(WEIR / "weir.14", WEIR / "weir.bcis")
+    write_adcirc(m, tmp_path / "w.14", tmp_path / "w.bcis", name="weir")
+    r = read_adcirc(tmp_path / "w.14", tmp_path / "w.bcis")
+    assert (tmp_path / "w.14").read_text().splitlines()[:2] == ["weir", "900 513"]
+    for k in ("node_xyz", "elt_nodes", "bound_edges", "bound_types", "frequency", "forcing_fact", "equilib_arg",
+              "node_amplitude", "node_phase", "levee_nodes", "levee_param"):
+        assert np.array_equal(getattr(r, k), getattr(m, k)), k
+    # without the .bcis the barrier columns of the .14 carry the same levee coefficients
+    r14 = read_adcirc(tmp_path / "w.14")
+    assert np.array_equal(r14.levee_param, m.levee_param) and np.array_equal(r14.bound_types, m.bound_types)
+    # one barrier segment (IBTYPE 4) of 19 pairs, the flow string (IBTYPE 2) of 19 nodes
+    lines = (tmp_path / "w.14").read_text().splitlines()
+    heads = [l for l in lines if "Number of nodes for land boundary" in l]
+    assert "19 4 = Number of nodes for land boundary %d" % len(heads) in heads
+    assert any(h.startswith("19 2 ") for h in heads)
+
+
+@pytest.mark.parametrize("pattern", [0, 1, 2])
+def test_generated_round_trip_all_codes(tmp_path, pattern):
+    m = quad_mesh(0.0, 0.0, 3000.0, 2000.0, 7, 5, bc=(BC_TIDE, BC_FLOW, BC_FUNCTION, BC_LAND), pattern=pattern,
+                  bathy="fourier", bmin=5.0, bmax=40.0, n_modes=4, jitter=0.2, seed=11, frequency=1.4e-4,
+                  forcing_fact=1.0, equilib_arg=3.0, amplitude=0.7, phase=12.5)
+    write_adcirc(m, tmp_path / "g.14", tmp_path / "g.bcis")
+    r = read_adcirc(tmp_path / "g.14", tmp_path / "g.bcis")
+    assert np.array_equal(r.node_xyz, m.node_xyz) and np.array_equal(r.elt_nodes, m.elt_nodes)
+    assert edge_set(r) == edge_set(m)
+    assert {t for *_, t in edge_set(m)} == {BC_TIDE, BC_FLOW, BC_FUNCTION, BC_LAND}
+    assert np.array_equal(r.frequency, m.frequency) and np.array_equal(r.equilib_arg, m.equilib_arg)
+    forced = np.unique(m.bound_edges[np.isin(m.bound_types, [BC_TIDE, BC_FLOW])])
+    assert np.array_equal(r.node_amplitude[forced], m.node_amplitude[forced])
+    assert np.array_equal(r.node_phase[forced], m.node_phase[forced])
+
+
+def test_oracle_steps_identical_after_round_trip(tmp_path):
+    m = quad_mesh(0.0, 0.0, 20000.0, 20000.0, 8, 8, bc=(BC_LAND, BC_FLOW, BC_LAND, BC_TIDE), pattern=1,
+                  bathy="fourier", bmin=5.0, bmax=50.0, n_modes=4, jitter=0.2, seed=3, frequency=1.405189e-4,
+                  forcing_fact=1.0, equilib_arg=0.0, amplitude=0.3, phase=20.0)
+    write_adcirc(m, tmp_path / "t.14", tmp_path / "t.bcis")
+    r = read_adcirc(tmp_path / "t.14", tmp_path / "t.bcis")
+    assert not np.array_equal(r.bound_edges, m.bound_edges)  # same edges, listed in another order
+    assert edge_set(r) == edge_set(m)
+    prm = make_params(p=2, nstages=3, order=3, dt=2.0, g=9.81, ramp_duration=0.001, ic_constant=True)
+    a, b = OracleSim(m, prm), OracleSim(r, prm)
+    a.step(30), b.step(30)
+    qa, qb = a.state()["q"], b.state()["q"]
+    assert np.abs(qa).max() > 0 and np.array_equal(qa, qb)
+    a.close(), b.close()
+
+
+def test_levee_segments_split_and_errors(tmp_path):
+    m = read_adcirc(WEIR / "weir.14", WEIR / "weir.bcis")
+    # the pairs in reverse order are the same barrier walked backwards: one segment, same edges
+    rev = Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, m.bound_types, levee_nodes=m.levee_nodes[::-1],
+               levee_param=m.levee_param[::-1])
+    write_adcirc(rev, tmp_path / "rev.14")
+    r = read_adcirc(tmp_path / "rev.14")
+    assert np.array_equal(r.bound_types, m.bound_types)
+    assert sorted(map(tuple, r.levee_nodes.tolist())) == sorted(map(tuple, m.levee_nodes.tolist()))
+    # pairs out of order leave levee edges that join no two consecutive pairs
+    perm = np.r_[0:5, 10:19, 5:10]
+    bad = Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, m.bound_types, levee_nodes=m.levee_nodes[perm],
+               levee_param=m.levee_param[perm])
+    with pytest.raises(RkdgError, match="joins no two consecutive levee pairs"):
+        write_adcirc(bad, tmp_path / "bad.14")
+    # a boundary code with no ADCIRC type
+    bt = m.bound_types.copy()
+    bt[0] = 7
+    with pytest.raises(RkdgError, match="has no ADCIRC boundary type"):
+        write_adcirc(Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, bt), tmp_path / "x.14")
+    with pytest.raises(RkdgError, match="cannot open"):
+        write_adcirc(m, tmp_path / "no_dir" / "x.14")
