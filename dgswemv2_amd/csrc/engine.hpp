@@ -989,6 +989,18 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     int nbv[3], ed[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) nbv[f] = ldc(A.nb, ne, f, e), ed[f] = ldc(A.edge, ne, f, e);
+    // FP32: the first earlier state of the SSP-RK combination (q0 in every stage of SSP-RK2/RK3
+    // that has one) is loaded here with the element's own modes, so its latency hides behind the
+    // volume integral instead of stalling the update at the end (18 registers, within the budget
+    // of three waves per SIMD).  FP64 fast has no registers for it and loads it at the update.
+    constexpr bool kTerm0 = sizeof(Real) == 4;
+    Real t0[3][ND];
+    if (kTerm0 && A.nterm > 0) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int d = 0; d < ND; ++d) t0[k][d] = ldc(A.term_q[0], ne, k * ND + d, e);
+    }
     // FP32 loads neighbour modes one face ahead (face 0's before the volume integral, face f+1's
     // while face f computes; two faces' worth of registers live at most) and the edge geometry up
     // front.  FP64 has no registers to spare: it loads both at the face itself (occupancy 2
@@ -1276,7 +1288,7 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
             if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
             Real acc = A.cur_a * q[k][d] + cur_coef<ENS>(A, IS) * sol;
             for (int i = 0; i < A.nterm; ++i) {
-                Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                Real term = A.term_a[i] * ((kTerm0 && i == 0) ? t0[k][d] : ldc(A.term_q[i], ne, k * ND + d, e));
                 if (A.term_sol[i]) term += term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc += term;
             }
