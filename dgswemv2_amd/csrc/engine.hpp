@@ -155,6 +155,46 @@ RKDG_HD inline T ldc(const T* base, int n, int c, int i) {
 #endif
 }
 
+// FP64 columns c = 0..ncol-1 of an element-fastest array for the 64 consecutive elements
+// e0..e0+63 of a wave, copied HBM -> LDS by direct-to-LDS buffer loads (no registers held while
+// they are in flight): column c lands as 64 doubles at dst[64 c ..].  Every lane of the wave must
+// execute it: lane j fetches dword j of each 256-byte half of the column slice.  Elements past
+// n - 1 read element n - 1 (their values are never used).  The data may be read only after
+// lds_columns_wait().
+RKDG_HD inline void lds_columns_fetch(const double* base, int n, int ncol, int e0, double* dst) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const int lane = (int)__lane_id();
+    const int ea = e0 + (lane >> 1) < n ? e0 + (lane >> 1) : n - 1;
+    const int eb = e0 + 32 + (lane >> 1) < n ? e0 + 32 + (lane >> 1) : n - 1;
+    const int va = ea * 8 + (lane & 1) * 4, vb = eb * 8 + (lane & 1) * 4;
+    const __amdgpu_buffer_rsrc_t r = rkdg_rsrc(base);
+    for (int c = 0; c < ncol; ++c) {
+        const int soff = c * n * 8;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4, va,
+                                                 soff, 0, 0);
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(dst + 64 * c + 32), 4,
+                                                 vb, soff, 0, 0);
+    }
+#else
+    (void)base, (void)n, (void)ncol, (void)e0, (void)dst;
+#endif
+}
+
+RKDG_HD inline int lane_index() {
+#ifdef __HIP_DEVICE_COMPILE__
+    return (int)__lane_id();
+#else
+    return 0;
+#endif
+}
+
+// Waits for every outstanding vector-memory load of the wave, the direct-to-LDS ones included.
+RKDG_HD inline void lds_columns_wait() {
+#ifdef __HIP_DEVICE_COMPILE__
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+#endif
+}
+
 template <typename T>
 RKDG_HD inline void stc(T* base, int n, int c, int i, T v) {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -506,7 +546,8 @@ RKDG_HD RKDG_FORCE_INLINE inline void finish_element(const StageArgs<Real>& A, i
 // The fused stage of one element, reference arithmetic order (FP64 plain form).
 
 template <typename Real, int P, bool WD, bool CF, bool ENS>
-RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e, const InstScal<Real>& IS) {
+RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, const FaceTables<Real>& FT, int e, const InstScal<Real>& IS,
+                                                    const Real* t0_lds) {
     using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
@@ -781,7 +822,9 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
         }
     }
 
-    // ---- M^-1 and the SSP-RK combination (element.hpp:500-505, explicit_ssp_rk_stepper.hpp:67-80)
+    // ---- M^-1 and the SSP-RK combination (element.hpp:500-505, explicit_ssp_rk_stepper.hpp:67-80);
+    // the first earlier state may wait in LDS (t0_lds, fetched when the stage started)
+    if (t0_lds) lds_columns_wait();
     Real nq[3][ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -791,7 +834,8 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
             if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
             Real acc = Real(0.0);
             for (int i = 0; i < A.nterm; ++i) {
-                Real term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                Real term = A.term_a[i] * ((i == 0 && t0_lds) ? t0_lds[64 * (k * ND + d) + lane_index()]
+                                                               : ldc(A.term_q[i], ne, k * ND + d, e));
                 if (A.term_sol[i]) term = term + term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc = acc + term;
             }
@@ -969,7 +1013,7 @@ RKDG_HD inline Real fmac_tab(Real a, Real c, Real r) { return c != Real(0) ? fma
 
 template <typename Real, int P, bool WD, bool CF, bool ENS>
 RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& A, const Real* phie_lds, const Real* phiew_lds,
-                                       int e, const InstScal<Real>& IS) {
+                                       int e, const InstScal<Real>& IS, const Real* t0_lds) {
     using C = ConstTables<Real, P>;
     constexpr int ND = Sizes<P>::ND, NG = Sizes<P>::NG, NE = Sizes<P>::NE;
     const int ne = A.ne;
@@ -992,7 +1036,8 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     // FP32: the first earlier state of the SSP-RK combination (q0 in every stage of SSP-RK2/RK3
     // that has one) is loaded here with the element's own modes, so its latency hides behind the
     // volume integral instead of stalling the update at the end (18 registers, within the budget
-    // of three waves per SIMD).  FP64 fast has no registers for it and loads it at the update.
+    // of three waves per SIMD).  FP64 has no registers for it: the kernel fetches it into LDS
+    // (t0_lds) when the stage starts.
     constexpr bool kTerm0 = sizeof(Real) == 4;
     Real t0[3][ND];
     if (kTerm0 && A.nterm > 0) {
@@ -1279,6 +1324,7 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
         }
     }
 
+    if (t0_lds) lds_columns_wait();
     Real nq[3][ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
@@ -1288,7 +1334,9 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
             if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
             Real acc = A.cur_a * q[k][d] + cur_coef<ENS>(A, IS) * sol;
             for (int i = 0; i < A.nterm; ++i) {
-                Real term = A.term_a[i] * ((kTerm0 && i == 0) ? t0[k][d] : ldc(A.term_q[i], ne, k * ND + d, e));
+                Real term = A.term_a[i] * ((kTerm0 && i == 0) ? t0[k][d]
+                                           : (i == 0 && t0_lds) ? t0_lds[64 * (k * ND + d) + lane_index()]
+                                                                : ldc(A.term_q[i], ne, k * ND + d, e));
                 if (A.term_sol[i]) term += term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
                 acc += term;
             }
