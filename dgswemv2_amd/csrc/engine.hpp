@@ -169,7 +169,7 @@ RKDG_HD inline void lds_columns_fetch(const double* base, int n, int ncol, int e
     const int va = ea * 8 + (lane & 1) * 4, vb = eb * 8 + (lane & 1) * 4;
     const __amdgpu_buffer_rsrc_t r = rkdg_rsrc(base);
     for (int c = 0; c < ncol; ++c) {
-        const int soff = c * n * 8;
+        const int soff = (int)((unsigned)c * (unsigned)n * 8u);  // below 2^32 (model builder)
         __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4, va,
                                                  soff, 0, 0);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(dst + 64 * c + 32), 4,
