@@ -423,15 +423,15 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd_kernel(TrainArgs T) {
     store_rows(T.dz1, T.Bpad, p, g, d0, d1);
 }
 
-// Weight gradients over one slice of 256 samples (work-group = slice): wave 0 dW1 = dZ1 Xb^T
+// Weight gradients over one slice of slen samples (work-group = slice): wave 0 dW1 = dZ1 Xb^T
 // (64 x 48, columns 36.. are zero padding), wave 1 dW2 = dZ2 H1^T, wave 2 dW3 = dZ3 H2^T, each as
 // MFMA products whose K is the sample; bias gradients are the row sums of the bf16 dZ; wave 3
 // dW4 = sum dz h3, db4 = sum dz and the slice's loss.  Partial sums go to part[slice][NPARAM]
 // and loss_part[slice] and are added in slice order by the update kernel (deterministic).
-__global__ void __launch_bounds__(256) mlp_wgrad_kernel(TrainArgs T, float* part, float* loss_part) {
+__global__ void __launch_bounds__(256) mlp_wgrad_kernel(TrainArgs T, float* part, float* loss_part, int slen) {
     const int lane = threadIdx.x & 63, g = lane >> 4, n = lane & 15, role = threadIdx.x >> 6;
-    const int slice = blockIdx.x, k_begin = slice * 256;
-    const int k_end = k_begin + 256 < T.Bpad ? k_begin + 256 : T.Bpad;
+    const int slice = blockIdx.x, k_begin = slice * slen;
+    const int k_end = k_begin + slen < T.Bpad ? k_begin + slen : T.Bpad;
     float* out = part + (size_t)slice * mlp::NPARAM;
     if (role < 3) {
         const uint16_t* A = role == 0 ? T.dz1 : role == 1 ? T.dz2 : T.dz3;
@@ -481,8 +481,15 @@ __global__ void __launch_bounds__(256) mlp_wgrad_kernel(TrainArgs T, float* part
             if (g == 0) out[offb + 16 * mt + n] = s;
         }
     } else {
-        float w4 = 0.0f;
-        for (int k = k_begin; k < k_end; ++k) w4 += T.dz[k] * T.h3[(long long)k * mlp::NH + lane];
+        // eight independent partial sums, so eight sample rows are in flight at a time (a single
+        // dependent chain waits out one load latency per sample); combined in a fixed order
+        float w8[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        for (int k = k_begin; k < k_end; k += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (k + u < k_end) w8[u] += T.dz[k + u] * T.h3[(long long)(k + u) * mlp::NH + lane];
+        }
+        const float w4 = ((w8[0] + w8[1]) + (w8[2] + w8[3])) + ((w8[4] + w8[5]) + (w8[6] + w8[7]));
         out[mlp::OFF_W4 + lane] = w4;
         float db = 0.0f, ls = 0.0f;
         for (int k = k_begin + lane; k < k_end; k += 64) db += T.dz[k], ls += T.loss[k];
@@ -535,8 +542,8 @@ hipError_t launch_mlp_fwdbwd(const TrainArgs& T, bool backward, hipStream_t s) {
 
 hipError_t launch_mlp_wgrad(const TrainArgs& T, float* part, float* loss_part, hipStream_t s) {
     if (T.Bpad % 32) return hipErrorInvalidValue;
-    const int nslices = (T.Bpad + 255) / 256;
-    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(nslices), dim3(256), 0, s, T, part, loss_part);
+    hipLaunchKernelGGL(mlp_wgrad_kernel, dim3(mlp::wgrad_slices(T.Bpad)), dim3(256), 0, s, T, part, loss_part,
+                       mlp::wgrad_slice(T.Bpad));
     return hipGetLastError();
 }
 

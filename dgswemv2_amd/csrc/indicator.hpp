@@ -44,6 +44,16 @@ constexpr int PACK = 4 * 2 * 64 * 8;  // bf16 entries of one packed A-operand ma
 constexpr int NPACKED = 5;
 // small per-layer vectors the kernels read from LDS: b1 b2 b3 w4 b4 threshold
 constexpr int NVEC = 4 * NH + 2;
+// Samples per weight-gradient slice (one work-group each, partial sums combined in slice order):
+// 64 for training batches of up to 4096 samples (more work-groups in flight), 256 beyond (bounds
+// the partial-gradient buffer of large gradient evaluations).
+constexpr int wgrad_slice(int bpad) { return bpad <= 4096 ? 64 : 256; }
+constexpr int wgrad_slices(int bpad) { return (bpad + wgrad_slice(bpad) - 1) / wgrad_slice(bpad); }
+// Slices any batch of at most bpad samples may need (the partial-gradient buffer's size).
+constexpr int wgrad_slices_cap(int bpad) {
+    const int a = wgrad_slices(bpad < 4096 ? bpad : 4096), b = wgrad_slices(bpad);
+    return a > b ? a : b;
+}
 constexpr double H_FLOOR = 0.01;
 // A write-back "matters" when it changes the state by more than this share of the physical scale:
 // 1e-8 for the FP64 forms; 1e-6 for FP32 bulk, the level of that form's own rounding per step.

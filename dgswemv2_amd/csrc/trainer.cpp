@@ -70,7 +70,7 @@ struct rkdg_mlp_trainer {
         for (void* p : {(void*)xb, (void*)h1, (void*)h2, (void*)dz1, (void*)dz2, (void*)dz3, (void*)h3, (void*)dz,
                         (void*)loss, (void*)logit, (void*)part, (void*)loss_part})
             if (p) release(p);
-        const size_t b = (size_t)bpad, ns = (b + 255) / 256;
+        const size_t b = (size_t)bpad, ns = (size_t)rkdg::mlp::wgrad_slices_cap(bpad);
         if (!alloc(xb, 48 * b) || !alloc(h1, 64 * b) || !alloc(h2, 64 * b) || !alloc(dz1, 64 * b) ||
             !alloc(dz2, 64 * b) || !alloc(dz3, 64 * b) || !alloc(h3, 64 * b) || !alloc(dz, b) || !alloc(loss, b) ||
             !alloc(logit, b) || !alloc(part, ns * NPARAM) || !alloc(loss_part, ns))
@@ -179,7 +179,7 @@ rkdg_status rkdg_mlp_train(rkdg_mlp_trainer* t, const rkdg_train_params* prm, fl
         for (int s = 0; s < steps && e == hipSuccess; ++s) {
             const int b0 = s * batch, B = std::min(batch, n - b0);
             const rkdg::TrainArgs T = t->args(t->X, t->cap, t->y, t->idx + b0, B, prm->pos_weight);
-            const int nslices = (T.Bpad + 255) / 256;
+            const int nslices = rkdg::mlp::wgrad_slices(T.Bpad);
             ++t->t;
             const float c1 = (float)(1.0 / (1.0 - std::pow((double)prm->beta1, (double)t->t)));
             const float c2 = (float)(1.0 / (1.0 - std::pow((double)prm->beta2, (double)t->t)));
@@ -232,7 +232,7 @@ rkdg_status rkdg_mlp_gradient(rkdg_mlp_trainer* t, const float* x, const float* 
         return terr(t, "non-finite samples");
     if (!t->ensure_batch(rkdg_mlp_trainer::pad(n)) || !t->stage_in(x, y, n)) return RKDG_ERR_HIP;
     const rkdg::TrainArgs T = t->args(t->Xs, t->cap, t->ys, t->ident, n, pos_weight);
-    const int nslices = (T.Bpad + 255) / 256;
+    const int nslices = rkdg::mlp::wgrad_slices(T.Bpad);
     float lsum = 0.0f;
     hipError_t e = rkdg::launch_mlp_fwdbwd(T, true, t->stream);
     if (e == hipSuccess) e = rkdg::launch_mlp_wgrad(T, t->part, t->loss_part, t->stream);
