@@ -140,7 +140,7 @@ def learned_bench(steps, scale, warm=40, judge_steps=10):
     collect_s = time.perf_counter() - t0
     log("collected", y.size, "samples from", nst, "limiter stages in", round(collect_s, 2), "s")
     t0 = time.perf_counter()
-    w, thr, info = IND.train(X, y, epochs=6, batch=1024, pos_weight=8.0, recall=0.9999, margin=6.0)
+    w, thr, info = IND.train(X, y, epochs=6, batch=1024, pos_weight=8.0, recall=0.9999, margin=16.0)
     train_s = time.perf_counter() - t0
     del X, y
     log("trained in", round(train_s, 2), "s:", json.dumps(info))

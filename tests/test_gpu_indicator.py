@@ -187,7 +187,7 @@ def test_learned_indicator_gain():
     assert np.all(np.isfinite(X))
     b.step(40)
     assert np.array_equal(a.state()["q"], b.state()["q"])
-    w, thr, info = IND.train(X, y, epochs=6, batch=1024, pos_weight=8.0, recall=0.9999, margin=6.0)
+    w, thr, info = IND.train(X, y, epochs=6, batch=1024, pos_weight=8.0, recall=0.9999, margin=16.0)
     banner("training:", info)
     b.set_indicator(w, threshold=thr, judge=True)
     a.step(10)
