@@ -17,7 +17,7 @@ from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
+    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "read_meta", "write_meta", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
@@ -163,6 +163,29 @@ def write_adcirc(mesh, fort14, bcis=None, name="mesh"):
                                     None if bcis is None else str(bcis).encode())
     if st != N.RKDG_OK:
         raise RkdgError(st, lib.rkdg_create_error().decode())
+
+
+def write_meta(mesh, path, name="mesh"):
+    """Writes a Mesh as a MeshMetaData (.meta) file (the reference partitioner's submesh format)."""
+    lib = N.load_library()
+    st = lib.rkdg_mesh_write_meta(C.byref(mesh.desc), name.encode(), str(path).encode())
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
+
+
+def read_meta(path, bcis=None):
+    """Reads a MeshMetaData (.meta) file and, optionally, its .bcis boundary data."""
+    lib = N.load_library()
+    handle = C.c_void_p()
+    desc = N.MeshDesc()
+    st = lib.rkdg_mesh_read_meta(str(path).encode(), None if bcis is None else str(bcis).encode(),
+                                 C.byref(handle), C.byref(desc))
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
+    try:
+        return _mesh_from_desc(desc)
+    finally:
+        lib.rkdg_mesh_free(handle)
 
 
 def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, problem=PROBLEM_NONE,

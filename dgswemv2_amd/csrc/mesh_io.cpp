@@ -158,6 +158,80 @@ int ibtype_of(const Fort14& F, long a, long b) {
     throw ReadError{"boundary not found for node pair (" + std::to_string(a) + ", " + std::to_string(b) + ")"};
 }
 
+// SWE::Inputs::read_bcis (problem_input/swe_inputs.cpp:212-312): tide / flow blocks (constituents
+// with amplitude and phase per node) and the levee block (coefficients per front/back node pair).
+// `node` maps a file node ID to its index.
+struct Bcis {
+    std::vector<std::array<double, 3>> cons;                       // frequency, forcing factor, eq. argument
+    std::map<long, std::vector<std::array<double, 2>>> forcing;    // node -> (amplitude, phase) per constituent
+    std::map<std::pair<long, long>, std::array<double, 3>> levee;  // (front, back) -> H, C_sub, C_super
+    std::vector<std::pair<long, long>> levee_order;                // pairs in file order
+
+    void apply(rkdg_mesh& M, long nn) const {
+        if (cons.empty()) return;
+        const size_t nc = cons.size();
+        for (auto& c : cons) M.freq.push_back(c[0]), M.ff.push_back(c[1]), M.eq.push_back(c[2]);
+        M.amp.assign((size_t)nn * nc, 0.0);
+        M.phase.assign((size_t)nn * nc, 0.0);
+        for (auto& kv : forcing) {
+            if (kv.first < 0 || kv.first >= nn) throw ReadError{"forcing node out of range"};
+            for (size_t c = 0; c < nc; ++c)
+                M.amp[(size_t)kv.first * nc + c] = kv.second[c][0], M.phase[(size_t)kv.first * nc + c] = kv.second[c][1];
+        }
+    }
+};
+
+template <typename NodeFn>
+Bcis read_bcis(const char* path, NodeFn&& node) {
+    std::ifstream ifs(path);
+    if (!ifs) throw ReadError{std::string("BC/ISP data file ") + path + " was not found"};
+    Bcis B;
+    std::string line;
+    while (std::getline(ifs, line)) {
+        long btype = -1, nnodes = 0;
+        std::istringstream(line) >> btype >> nnodes;
+        if (btype == RKDG_BC_TIDE || btype == RKDG_BC_FLOW) {
+            long ncon = 0;
+            next_line(ifs, "constituent count") >> ncon;
+            std::vector<std::array<double, 3>> here;
+            std::map<long, std::vector<std::array<double, 2>>> block;
+            for (long c = 0; c < ncon; ++c) {
+                std::array<double, 3> cc;
+                next_line(ifs, "constituent") >> cc[0] >> cc[1] >> cc[2];
+                here.push_back(cc);
+                for (long i = 0; i < nnodes; ++i) {
+                    long nd;
+                    double amp, ph;
+                    auto ss = next_line(ifs, "forcing node");
+                    if (!(ss >> nd >> amp >> ph)) throw ReadError{"malformed forcing line"};
+                    block[node(nd)].push_back({amp, ph});
+                }
+            }
+            if (B.cons.empty())
+                B.cons = here;
+            else if (B.cons != here)
+                throw ReadError{"tide/flow blocks with different constituent lists are not supported"};
+            for (auto& kv : block) {
+                auto it = B.forcing.find(kv.first);
+                if (it != B.forcing.end() && it->second != kv.second)
+                    throw ReadError{"node " + std::to_string(kv.first) + " forced by two different blocks"};
+                B.forcing[kv.first] = kv.second;
+            }
+        } else if (btype == RKDG_BC_LEVEE) {
+            for (long i = 0; i < nnodes; ++i) {
+                long f, b;
+                std::array<double, 3> v;
+                auto ss = next_line(ifs, "levee node pair");
+                if (!(ss >> f >> b >> v[0] >> v[1] >> v[2])) throw ReadError{"malformed levee line"};
+                const std::pair<long, long> key{node(f), node(b)};
+                if (!B.levee.count(key)) B.levee_order.push_back(key);
+                B.levee[key] = v;
+            }
+        }
+    }
+    return B;
+}
+
 }  // namespace
 
 extern "C" rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char* bcis_path, rkdg_mesh** out,
@@ -219,65 +293,10 @@ extern "C" rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char
             }
 
         // .bcis: tide/flow constituents per node, levee coefficients per node pair
-        std::map<std::pair<long, long>, std::array<double, 3>> levee_bcis;
-        std::vector<std::array<double, 3>> cons;
-        std::map<long, std::vector<std::array<double, 2>>> node_forcing;  // node -> (amplitude, phase) per constituent
-        if (bcis_path) {
-            std::ifstream ifs(bcis_path);
-            if (!ifs) throw ReadError{std::string("BC/ISP data file ") + bcis_path + " was not found"};
-            std::string line;
-            while (std::getline(ifs, line)) {
-                long btype = -1, nnodes = 0;
-                std::istringstream(line) >> btype >> nnodes;
-                if (btype == RKDG_BC_TIDE || btype == RKDG_BC_FLOW) {
-                    long ncon = 0;
-                    next_line(ifs, "constituent count") >> ncon;
-                    std::vector<std::array<double, 3>> here;
-                    std::map<long, std::vector<std::array<double, 2>>> block;
-                    for (long c = 0; c < ncon; ++c) {
-                        std::array<double, 3> cc;
-                        next_line(ifs, "constituent") >> cc[0] >> cc[1] >> cc[2];
-                        here.push_back(cc);
-                        for (long i = 0; i < nnodes; ++i) {
-                            long nd;
-                            double amp, ph;
-                            auto ss = next_line(ifs, "forcing node");
-                            if (!(ss >> nd >> amp >> ph)) throw ReadError{"malformed forcing line"};
-                            block[node(nd)].push_back({amp, ph});
-                        }
-                    }
-                    if (cons.empty())
-                        cons = here;
-                    else if (cons != here)
-                        throw ReadError{"tide/flow blocks with different constituent lists are not supported"};
-                    for (auto& kv : block) {
-                        auto it = node_forcing.find(kv.first);
-                        if (it != node_forcing.end() && it->second != kv.second)
-                            throw ReadError{"node " + std::to_string(kv.first) + " forced by two different blocks"};
-                        node_forcing[kv.first] = kv.second;
-                    }
-                } else if (btype == RKDG_BC_LEVEE) {
-                    for (long i = 0; i < nnodes; ++i) {
-                        long f, b;
-                        std::array<double, 3> v;
-                        auto ss = next_line(ifs, "levee node pair");
-                        if (!(ss >> f >> b >> v[0] >> v[1] >> v[2])) throw ReadError{"malformed levee line"};
-                        levee_bcis[{node(f), node(b)}] = v;
-                    }
-                }
-            }
-        }
-        if (!cons.empty()) {
-            const size_t nc = cons.size();
-            for (auto& c : cons) M->freq.push_back(c[0]), M->ff.push_back(c[1]), M->eq.push_back(c[2]);
-            M->amp.assign((size_t)nn * nc, 0.0);
-            M->phase.assign((size_t)nn * nc, 0.0);
-            for (auto& kv : node_forcing) {
-                if (kv.first < 0 || kv.first >= nn) throw ReadError{"forcing node out of range"};
-                for (size_t c = 0; c < nc; ++c)
-                    M->amp[(size_t)kv.first * nc + c] = kv.second[c][0], M->phase[(size_t)kv.first * nc + c] = kv.second[c][1];
-            }
-        }
+        Bcis B;
+        if (bcis_path) B = read_bcis(bcis_path, node);
+        B.apply(*M, nn);
+        const auto& levee_bcis = B.levee;
         // barrier node pairs (front NBVV, back IBCONN); coefficients from the .bcis levee block
         // (what the reference uses, swe_pre_create_intface.hpp:50-69), else from the .14 barrier
         // columns BARINTH / BARINCFSB / BARINCFSP
@@ -496,6 +515,202 @@ extern "C" rkdg_status rkdg_mesh_write_adcirc(const rkdg_mesh_desc* d, const cha
         }
         return RKDG_OK;
     } catch (const WriteError& err) {
+        rkdg::set_create_error(err.msg);
+    } catch (const std::bad_alloc&) {
+        rkdg::set_create_error("out of host memory");
+    }
+    return RKDG_ERR_ARG;
+}
+
+// ---- MeshMetaData (.meta) files -------------------------------------------------------------
+// Layout of MeshMetaData::write_to / MeshMetaData(const std::string&)
+// (preprocessor/mesh_metadata.cpp:87-137, element and node records mesh_metadata.hpp:11-72): the
+// mesh name, "<n> = number of elements", one line per element "ID nfaces node0 node1 node2
+// neighbour0 neighbour1 neighbour2 type0 type1 type2" with face k opposite node k, the neighbour
+// 4294967295 on a boundary face (the element across the barrier on a levee face) and each type
+// one raw byte (SWE::BoundaryTypes: 0 land, 1 tide, 2 flow, 3 function, 4 outflow, 128 interior,
+// 129 levee), then "<n> = number of nodes" and one line per node "ID x y z".
+
+namespace {
+
+constexpr unsigned kNoNeighbor = 4294967295u;  // DEFAULT_ID (general_definitions.hpp:230)
+constexpr int kInternal = 128;                 // INTERNAL (utilities/edge_types.hpp:4)
+
+uint64_t edge_key(long a, long b) { return (uint64_t)std::min(a, b) << 32 | (uint64_t)std::max(a, b); }
+
+}  // namespace
+
+extern "C" rkdg_status rkdg_mesh_write_meta(const rkdg_mesh_desc* d, const char* name, const char* path) {
+    if (!d || !path) {
+        rkdg::set_create_error("null argument");
+        return RKDG_ERR_ARG;
+    }
+    try {
+        const long nn = d->n_nodes, ne = d->n_elements, nb = d->n_bound_edges, nl = d->n_levee_pairs;
+        if (nn <= 0 || ne <= 0 || !d->node_xyz || !d->elt_nodes) throw WriteError{"empty mesh"};
+        if (nb < 0 || (nb > 0 && (!d->bound_edge_nodes || !d->bound_edge_type))) throw WriteError{"bad boundary edges"};
+        if (nl < 0 || (nl > 0 && !d->levee_nodes)) throw WriteError{"bad levee pairs"};
+        for (long i = 0; i < 3 * ne; ++i)
+            if (d->elt_nodes[i] < 0 || d->elt_nodes[i] >= nn) throw WriteError{"element node out of range"};
+        // faces by edge: the (up to two) elements holding it
+        std::unordered_map<uint64_t, std::vector<std::pair<long, int>>> faces;
+        faces.reserve(3 * (size_t)ne);
+        for (long e = 0; e < ne; ++e)
+            for (int k = 0; k < 3; ++k)
+                faces[edge_key(d->elt_nodes[3 * e + (k + 1) % 3], d->elt_nodes[3 * e + (k + 2) % 3])].push_back({e, k});
+        std::unordered_map<uint64_t, int> btype;
+        for (long i = 0; i < nb; ++i)
+            btype[edge_key(d->bound_edge_nodes[2 * i], d->bound_edge_nodes[2 * i + 1])] = d->bound_edge_type[i];
+        // levee partner of a node: front <-> back (AdcircFormat::get_internal_node_pair)
+        std::unordered_map<long, long> partner;
+        for (long i = 0; i < nl; ++i) {
+            partner[d->levee_nodes[2 * i]] = d->levee_nodes[2 * i + 1];
+            partner[d->levee_nodes[2 * i + 1]] = d->levee_nodes[2 * i];
+        }
+        FILE* f = std::fopen(path, "wb");
+        if (!f) throw WriteError{std::string("cannot open ") + path};
+        std::unique_ptr<FILE, int (*)(FILE*)> guard(f, &std::fclose);
+        std::fprintf(f, "%s\n%ld = number of elements\n", name && *name ? name : "mesh", ne);
+        for (long e = 0; e < ne; ++e) {
+            unsigned nbr[3];
+            int type[3];
+            for (int k = 0; k < 3; ++k) {
+                const long a = d->elt_nodes[3 * e + (k + 1) % 3], b = d->elt_nodes[3 * e + (k + 2) % 3];
+                const auto& fl = faces[edge_key(a, b)];
+                if (fl.size() == 2) {
+                    nbr[k] = (unsigned)(fl[0].first == e ? fl[1].first : fl[0].first), type[k] = kInternal;
+                    continue;
+                }
+                if (fl.size() != 1) throw WriteError{"edge shared by more than two elements"};
+                auto it = btype.find(edge_key(a, b));
+                if (it == btype.end())
+                    throw WriteError{"boundary face (" + std::to_string(a) + ", " + std::to_string(b) + ") has no code"};
+                type[k] = it->second, nbr[k] = kNoNeighbor;
+                if (type[k] == RKDG_BC_LEVEE) {
+                    auto pa = partner.find(a), pb = partner.find(b);
+                    if (pa == partner.end() || pb == partner.end())
+                        throw WriteError{"levee face (" + std::to_string(a) + ", " + std::to_string(b) +
+                                         ") has no levee node pairs"};
+                    auto fo = faces.find(edge_key(pa->second, pb->second));
+                    if (fo == faces.end() || fo->second.size() != 1)
+                        throw WriteError{"levee face (" + std::to_string(a) + ", " + std::to_string(b) +
+                                         ") has no face across the barrier"};
+                    nbr[k] = (unsigned)fo->second[0].first;
+                }
+            }
+            std::fprintf(f, "%ld 3 %d %d %d %u %u %u %c %c %c\n", e, d->elt_nodes[3 * e], d->elt_nodes[3 * e + 1],
+                         d->elt_nodes[3 * e + 2], nbr[0], nbr[1], nbr[2], (char)type[0], (char)type[1], (char)type[2]);
+        }
+        std::fprintf(f, "%ld = number of nodes\n", nn);
+        for (long i = 0; i < nn; ++i)
+            std::fprintf(f, "%ld %.17g %.17g %.17g\n", i, d->node_xyz[3 * i], d->node_xyz[3 * i + 1], d->node_xyz[3 * i + 2]);
+        if (std::ferror(f)) throw WriteError{std::string("write error on ") + path};
+        return RKDG_OK;
+    } catch (const WriteError& err) {
+        rkdg::set_create_error(err.msg);
+    } catch (const std::bad_alloc&) {
+        rkdg::set_create_error("out of host memory");
+    }
+    return RKDG_ERR_ARG;
+}
+
+extern "C" rkdg_status rkdg_mesh_read_meta(const char* meta_path, const char* bcis_path, rkdg_mesh** out,
+                                           rkdg_mesh_desc* desc) {
+    if (!meta_path || !out || !desc) {
+        rkdg::set_create_error("null argument");
+        return RKDG_ERR_ARG;
+    }
+    *out = nullptr;
+    try {
+        std::ifstream ifs(meta_path, std::ios::binary);
+        if (!ifs) throw ReadError{std::string("meta mesh file ") + meta_path + " was not found"};
+        std::string name;
+        ifs >> name;
+        ifs.ignore(1 << 20, '\n');
+        long ne = 0, nn = 0;
+        ifs >> ne;
+        ifs.ignore(1 << 20, '\n');
+        if (!ifs || ne <= 0) throw ReadError{"bad element count"};
+        struct Elt {
+            std::array<long, 3> node;
+            std::array<unsigned char, 3> type;
+        };
+        std::map<long, Elt> elts;
+        for (long i = 0; i < ne; ++i) {
+            long id, nf;
+            Elt E;
+            unsigned long nbr;
+            ifs >> id >> nf;
+            if (!ifs || nf != 3) throw ReadError{"malformed element record (only triangles are supported)"};
+            for (int k = 0; k < 3; ++k) ifs >> E.node[k];
+            for (int k = 0; k < 3; ++k) ifs >> nbr;
+            for (int k = 0; k < 3; ++k) {
+                char c;
+                ifs >> c;  // one raw byte per face (skips whitespace, as operator>> on unsigned char does)
+                E.type[k] = (unsigned char)c;
+            }
+            ifs.ignore(1 << 20, '\n');
+            if (!ifs) throw ReadError{"malformed element record"};
+            if (!elts.emplace(id, E).second) throw ReadError{"element defined twice"};
+        }
+        ifs >> nn;
+        ifs.ignore(1 << 20, '\n');
+        if (!ifs || nn <= 0) throw ReadError{"bad node count"};
+        std::map<long, std::array<double, 3>> nodes;
+        for (long i = 0; i < nn; ++i) {
+            long id;
+            std::array<double, 3> v;
+            ifs >> id >> v[0] >> v[1] >> v[2];
+            ifs.ignore(1 << 20, '\n');
+            if (!ifs) throw ReadError{"malformed node record"};
+            if (!nodes.emplace(id, v).second) throw ReadError{"node defined twice"};
+        }
+        // IDs become their rank among the file's IDs (order-preserving, as for the .14)
+        std::unordered_map<long, long> nrank;
+        std::unique_ptr<rkdg_mesh> M(new rkdg_mesh());
+        M->xyz.resize(3 * (size_t)nn);
+        for (const auto& kv : nodes) {
+            const long r = (long)nrank.size();
+            nrank[kv.first] = r;
+            for (int k = 0; k < 3; ++k) M->xyz[3 * (size_t)r + k] = kv.second[k];
+        }
+        auto node = [&](long id) {
+            auto it = nrank.find(id);
+            if (it == nrank.end()) throw ReadError{"reference to undefined node " + std::to_string(id)};
+            return it->second;
+        };
+        M->elt.reserve(3 * (size_t)ne);
+        for (const auto& kv : elts)
+            for (int k = 0; k < 3; ++k) M->elt.push_back((int32_t)node(kv.second.node[k]));
+        // boundary faces in element, then face order (the order the .14 reader lists them in)
+        long e = 0;
+        for (const auto& kv : elts) {
+            for (int k = 0; k < 3; ++k) {
+                const int t = kv.second.type[k];
+                if (t == kInternal) continue;
+                if (t > RKDG_BC_LEVEE || (t > 4 && t < kInternal))
+                    throw ReadError{"undefined boundary type " + std::to_string(t) + " in meta mesh"};
+                M->bedge.push_back(M->elt[3 * e + (k + 1) % 3]), M->bedge.push_back(M->elt[3 * e + (k + 2) % 3]);
+                M->btype.push_back((uint8_t)t);
+            }
+            ++e;
+        }
+        Bcis B;
+        if (bcis_path) B = read_bcis(bcis_path, node);
+        B.apply(*M, nn);
+        // levee pairs and coefficients: the .bcis levee block (the .meta holds no node pairs)
+        for (const auto& key : B.levee_order) {
+            const auto& v = B.levee.at(key);
+            M->levee_nodes.push_back((int32_t)key.first), M->levee_nodes.push_back((int32_t)key.second);
+            for (int k = 0; k < 3; ++k) M->levee_param.push_back(v[k]);
+        }
+        if (M->levee_nodes.empty())
+            for (uint8_t t : M->btype)
+                if (t == RKDG_BC_LEVEE) throw ReadError{"levee faces need the .bcis levee block"};
+        M->fill(desc);
+        *out = M.release();
+        return RKDG_OK;
+    } catch (const ReadError& err) {
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");

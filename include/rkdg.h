@@ -287,6 +287,18 @@ void rkdg_mesh_free(rkdg_mesh* mesh);
 rkdg_status rkdg_mesh_write_adcirc(const rkdg_mesh_desc* mesh, const char* name, const char* fort14_path,
                                    const char* bcis_path);
 
+/* MeshMetaData files (.meta, the partitioner's per-submesh mesh format): the layout of
+ * MeshMetaData::write_to and MeshMetaData(const std::string&) (preprocessor/mesh_metadata.cpp:
+ * 87-137, records mesh_metadata.hpp:11-72): element records "ID 3 nodes neighbours types" with
+ * face k opposite node k, neighbour 4294967295 on boundary faces (the element across the barrier
+ * on levee faces) and each face type one raw byte (SWE::BoundaryTypes, 128 = interior), then node
+ * records "ID x y z".  rkdg_mesh_write_meta prints reals with 17 significant digits; IDs are
+ * indices.  rkdg_mesh_read_meta relabels IDs by rank (as rkdg_mesh_read_adcirc), lists boundary
+ * faces in element and face order, and takes forcing and levee pairs from the .bcis (NULL: none;
+ * levee faces then are an error).  Errors: RKDG_ERR_ARG with the message in rkdg_create_error(). */
+rkdg_status rkdg_mesh_write_meta(const rkdg_mesh_desc* mesh, const char* name, const char* meta_path);
+rkdg_status rkdg_mesh_read_meta(const char* meta_path, const char* bcis_path, rkdg_mesh** out, rkdg_mesh_desc* desc);
+
 /* ---- output ----------------------------------------------------------------------------
  * The reference's writers, at a step boundary, single-mesh contexts.
  * rkdg_write_modal: write_modal_data (problem_postprocessor/swe_post_write_modal.hpp:5-90):

@@ -1,4 +1,5 @@
-"""ADCIRC .14 / .bcis writer (rkdg_mesh_write_adcirc) against the reader (rkdg_mesh_read_adcirc).
+"""ADCIRC .14 / .bcis writer (rkdg_mesh_write_adcirc) against the reader (rkdg_mesh_read_adcirc),
+and the MeshMetaData (.meta) writer / reader.
 
 The writer follows AdcircFormat::write_to (preprocessor/ADCIRC_reader/adcirc_format.cpp:151-212)
 and the block layout SWE::Inputs::read_bcis parses (problem_input/swe_inputs.cpp:212-312).  A
@@ -10,8 +11,10 @@ the re-read mesh bit-identical to those on the original.
 import numpy as np
 import pytest
 
+from pathlib import Path
+
 from dgswemv2_amd import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, Mesh, RkdgError, make_params, quad_mesh,
-                          read_adcirc, write_adcirc)
+                          read_adcirc, read_meta, write_adcirc, write_meta)
 from oracle.oracle_py import OracleSim
 from test_levee import WEIR
 
@@ -92,3 +95,89 @@ def test_levee_segments_split_and_errors(tmp_path):
         write_adcirc(Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, bt), tmp_path / "x.14")
     with pytest.raises(RkdgError, match="cannot open"):
         write_adcirc(m, tmp_path / "no_dir" / "x.14")
+
+
+# ---- MeshMetaData (.meta) files: MeshMetaData::write_to / MeshMetaData(file)
+# (preprocessor/mesh_metadata.cpp:87-137).  The reference's own check (test/test_mesh_metadata.cpp)
+# reads a .14, writes the .meta, reads it back and compares; the same on its sample_fort.14 fixture
+# (copied as data into tests/golden/) and on the weir.
+SAMPLE = WEIR.parent / "sample_fort.14"
+
+
+def meta_records(path):
+    """Element records of a .meta file as (nodes, neighbours, type bytes), read as bytes."""
+    lines = Path(path).read_bytes().split(b"\n")
+    ne = int(lines[1].split()[0])
+    recs = []
+    for ln in lines[2:2 + ne]:
+        f = ln.split(b" ")
+        assert f[1] == b"3" and len(f) == 11, ln
+        recs.append(([int(x) for x in f[2:5]], [int(x) for x in f[5:8]], [f[8][0] if f[8] else 32, f[9][0], f[10][0]]))
+    return recs
+
+
+def test_sample_fort14_meta_round_trip(tmp_path):
+    m = read_adcirc(SAMPLE)
+    assert m.n_elements == 200 and len(m.node_xyz) == 130
+    types, counts = np.unique(m.bound_types, return_counts=True)
+    assert dict(zip(types.tolist(), counts.tolist())) == {BC_LAND: 54, BC_TIDE: 4}
+    write_meta(m, tmp_path / "s.meta", name="sample")
+    r = read_meta(tmp_path / "s.meta")
+    for k in ("node_xyz", "elt_nodes", "bound_edges", "bound_types"):
+        assert np.array_equal(getattr(r, k), getattr(m, k)), k
+    # and the .14 writer on the same fixture
+    write_adcirc(m, tmp_path / "s.14")
+    r14 = read_adcirc(tmp_path / "s.14")
+    assert np.array_equal(r14.node_xyz, m.node_xyz) and np.array_equal(r14.bound_types, m.bound_types)
+
+
+def test_meta_records_match_mesh_adjacency(tmp_path):
+    m = read_adcirc(WEIR / "weir.14", WEIR / "weir.bcis")
+    write_meta(m, tmp_path / "w.meta", name="weir")
+    recs = meta_records(tmp_path / "w.meta")
+    assert len(recs) == m.n_elements
+    edges = {}
+    for e, nodes in enumerate(m.elt_nodes.tolist()):
+        for k in range(3):
+            edges.setdefault(frozenset((nodes[(k + 1) % 3], nodes[(k + 2) % 3])), []).append(e)
+    btype = {frozenset(map(int, be)): int(t) for be, t in zip(m.bound_edges, m.bound_types)}
+    partner = {}
+    for f, b in m.levee_nodes.tolist():
+        partner[f], partner[b] = b, f
+    nlev = 0
+    for e, (nodes, nbr, ty) in enumerate(recs):
+        assert nodes == m.elt_nodes[e].tolist()
+        for k in range(3):
+            key = frozenset((nodes[(k + 1) % 3], nodes[(k + 2) % 3]))
+            if len(edges[key]) == 2:
+                assert ty[k] == 128 and nbr[k] == [x for x in edges[key] if x != e][0]
+            elif btype[key] == BC_LEVEE:
+                a, b = tuple(key)
+                assert ty[k] == 129 and nbr[k] == edges[frozenset((partner[a], partner[b]))][0]
+                nlev += 1
+            else:
+                assert ty[k] == btype[key] and nbr[k] == 4294967295
+    assert nlev == 36
+
+
+def test_weir_meta_round_trip_and_oracle(tmp_path):
+    m = read_adcirc(WEIR / "weir.14", WEIR / "weir.bcis")
+    write_meta(m, tmp_path / "w.meta", name="weir")
+    r = read_meta(tmp_path / "w.meta", WEIR / "weir.bcis")
+    for k in ("node_xyz", "elt_nodes", "bound_edges", "bound_types", "frequency", "forcing_fact", "equilib_arg",
+              "node_amplitude", "node_phase"):
+        assert np.array_equal(getattr(r, k), getattr(m, k)), k
+    pairs = lambda x: sorted(zip(map(tuple, x.levee_nodes.tolist()), map(tuple, x.levee_param.tolist())))
+    assert pairs(r) == pairs(m)
+    prm = make_params(p=1, nstages=2, order=2, dt=3.0, slope_limiting=True, sl_M=1e-8, sl_nu=1.5, ze0=0.3)
+    a, b = OracleSim(m, prm), OracleSim(r, prm)
+    a.step(10), b.step(10)
+    assert np.array_equal(a.state()["q"], b.state()["q"])
+    a.close(), b.close()
+    with pytest.raises(RkdgError, match="levee faces need the .bcis levee block"):
+        read_meta(tmp_path / "w.meta")
+    bad = tmp_path / "bad.meta"
+    bad.write_bytes(b"x\n1 = number of elements\n0 3 0 1 2 4294967295 4294967295 4294967295 \x07 \x00 \x00\n"
+                    b"3 = number of nodes\n0 0 0 1\n1 1 0 1\n2 0 1 1\n")
+    with pytest.raises(RkdgError, match="undefined boundary type 7"):
+        read_meta(bad)
