@@ -17,7 +17,7 @@ from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "read_meta", "write_meta", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
+    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "read_meta", "write_meta", "partition_rcb", "submesh", "PartitionedSimulation", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
@@ -188,6 +188,108 @@ def read_meta(path, bcis=None):
         lib.rkdg_mesh_free(handle)
 
 
+def partition_rcb(mesh, nparts):
+    """Element -> part (recursive coordinate bisection of the barycentres, equal part sizes)."""
+    lib = N.load_library()
+    part = np.empty(mesh.n_elements, np.int32)
+    st = lib.rkdg_partition_rcb(C.byref(mesh.desc), nparts, N.iptr(part))
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
+    return part
+
+
+def submesh(mesh, part, p):
+    """Part p's submesh (its elements + ghost neighbours, global order) and the global index of
+    each local element."""
+    lib = N.load_library()
+    part = np.ascontiguousarray(part, dtype=np.int32)
+    handle, desc, gid = C.c_void_p(), N.MeshDesc(), C.POINTER(C.c_int32)()
+    st = lib.rkdg_submesh(C.byref(mesh.desc), N.iptr(part), p, C.byref(handle), C.byref(desc), C.byref(gid))
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
+    try:
+        return _mesh_from_desc(desc), np.ctypeslib.as_array(gid, shape=(desc.n_elements,)).copy()
+    finally:
+        lib.rkdg_mesh_free(handle)
+
+
+class PartitionedSimulation:
+    """A mesh split into parts (one context each, all on this GPU) stepped with a halo exchange
+    after every stage: the distributed-boundary path of the reference's MPI run, with the
+    exchange as device-side pack / unpack kernels (on several GPUs the buffers would travel
+    between devices, e.g. by RCCL send / receive).  Equals the whole-mesh run bit for bit."""
+
+    def __init__(self, mesh, params, precision=RKDG_FP64_PLAIN, nparts=2, part=None):
+        self.mesh, self.params, self.nparts = mesh, params, nparts
+        self.part = partition_rcb(mesh, nparts) if part is None else np.ascontiguousarray(part, np.int32)
+        self.sims, self.gids = [], []
+        try:
+            for p in range(nparts):
+                sub, gid = submesh(mesh, self.part, p)
+                self.sims.append(Simulation(sub, params, precision))
+                self.gids.append(gid)
+            # exchange plan: part q's ghosts owned by p, in global order, as local indices on both
+            # sides; slot q on p packs, slot 32 + p on q unpacks
+            if nparts > 32:
+                raise ValueError("at most 32 parts (64 halo slots per context)")
+            self.links = []
+            loc = [{int(g): i for i, g in enumerate(gid)} for gid in self.gids]
+            for q in range(nparts):
+                gq = self.gids[q]
+                own_q = self.part[gq]
+                for p in range(nparts):
+                    if p == q:
+                        continue
+                    g = gq[own_q == p]
+                    if g.size == 0:
+                        continue
+                    self.sims[p].halo_define(q, [loc[p][int(x)] for x in g])
+                    self.sims[q].halo_define(32 + p, [loc[q][int(x)] for x in g])
+                    self.links.append((p, q))
+            self.own = [self.part[gid] == p for p, gid in enumerate(self.gids)]
+            self.exchange()
+        except Exception:
+            self.close()
+            raise
+
+    def exchange(self):
+        for s in self.sims:
+            s.sync()
+        for p, q in self.links:
+            self.sims[p].halo_pack(q)
+        for s in self.sims:
+            s.sync()
+        for p, q in self.links:
+            self.sims[q].halo_unpack(32 + p, self.sims[p].halo_buffer(q)[0])
+        for s in self.sims:
+            s.sync()
+
+    def stage(self):
+        for s in self.sims:
+            s.stage()
+        self.exchange()
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps * self.params.nstages):
+            self.stage()
+
+    def state(self):
+        """The whole mesh's state (q, wet) gathered from the parts' own elements."""
+        ndof = (self.params.p + 1) * (self.params.p + 2) // 2
+        q = np.zeros((3, ndof, self.mesh.n_elements))
+        wet = np.zeros(self.mesh.n_elements, np.uint8)
+        for s, gid, own in zip(self.sims, self.gids, self.own):
+            st = s.state()
+            q[:, :, gid[own]] = st["q"][:, :, own]
+            wet[gid[own]] = st["wet"][own]
+        return {"q": q, "wet": wet}
+
+    def close(self):
+        for s in self.sims:
+            s.close()
+        self.sims = []
+
+
 def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, problem=PROBLEM_NONE,
                 ic_constant=True, ze0=0.0, qx0=0.0, qy0=0.0, function_source=False, friction=FRICTION_NONE,
                 friction_cf=0.0, wetting_drying=False, h_o=0.1, slope_limiting=False, sl_M=1.0e-8, sl_nu=1.5,
@@ -248,6 +350,22 @@ class Simulation:
 
     def sync(self):
         self._check(self.lib.rkdg_sync(self.ctx))
+
+    # halo exchange of a partitioned mesh (rkdg_halo_*)
+    def halo_define(self, slot, local_elements):
+        idx = np.ascontiguousarray(local_elements, dtype=np.int32)
+        self._check(self.lib.rkdg_halo_define(self.ctx, slot, N.iptr(idx) if idx.size else None, idx.size))
+
+    def halo_buffer(self, slot):
+        buf, nbytes = C.c_void_p(), C.c_int64()
+        self._check(self.lib.rkdg_halo_buffer(self.ctx, slot, C.byref(buf), C.byref(nbytes)))
+        return buf.value, nbytes.value
+
+    def halo_pack(self, slot):
+        self._check(self.lib.rkdg_halo_pack(self.ctx, slot))
+
+    def halo_unpack(self, slot, device_buffer):
+        self._check(self.lib.rkdg_halo_unpack(self.ctx, slot, C.c_void_p(device_buffer)))
 
     def state(self):
         """Modal state [3, ndof, ne] plus wet / limited [3, ne] / troubled flags."""

@@ -133,6 +133,13 @@ SIGNATURES = {
     "rkdg_mesh_read_adcirc": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
     "rkdg_mesh_write_adcirc": (C.c_int, [C.POINTER(MeshDesc), C.c_char_p, C.c_char_p, C.c_char_p]),
     "rkdg_mesh_write_meta": (C.c_int, [C.POINTER(MeshDesc), C.c_char_p, C.c_char_p]),
+    "rkdg_partition_rcb": (C.c_int, [C.POINTER(MeshDesc), C.c_int32, C.c_void_p]),
+    "rkdg_submesh": (C.c_int, [C.POINTER(MeshDesc), C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(MeshDesc),
+                               C.POINTER(C.POINTER(C.c_int32))]),
+    "rkdg_halo_define": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
+    "rkdg_halo_buffer": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
+    "rkdg_halo_pack": (C.c_int, [C.c_void_p, C.c_int32]),
+    "rkdg_halo_unpack": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
     "rkdg_mesh_read_meta": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
     "rkdg_create_batch": (C.c_int, [C.c_int32, C.POINTER(MeshDesc), C.POINTER(Params), C.c_int,
                                     C.POINTER(C.c_void_p)]),

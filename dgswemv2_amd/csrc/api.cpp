@@ -51,6 +51,10 @@ struct EngineBase {
     virtual rkdg_status indicator_stats(rkdg_indicator_stats* st, bool reset) = 0;
     virtual rkdg_status set_capture(bool on) = 0;
     virtual rkdg_status get_capture(float* feat, uint8_t* label, uint8_t* ok, float* logit, float* change) = 0;
+    // halo exchange of a partitioned mesh
+    virtual rkdg_status halo_define(int slot, const int32_t* idx, int n) = 0;
+    virtual rkdg_status halo_buffer(int slot, void** buf, int64_t* bytes) = 0;
+    virtual rkdg_status halo_move(int slot, bool pack, const void* src) = 0;
     int block = 128;            // work-items per block (measured best for the stage kernels on 1M elements)
     int persistent_grid = 0;    // > 0: persistent stage kernel on this many work-groups (ensembles)
     bool capturing = false;     // inside a graph capture: no host-side event records
@@ -117,6 +121,47 @@ struct Engine final : EngineBase {
         if (const char* b = std::getenv("RKDG_BLOCK")) block = std::atoi(b);  // tuning experiments
         if (block < 64 || block > 256 || block % 64) block = 128;
     }
+    // halo slots: listed local elements and a [3 ndof + 1][n] buffer each
+    struct Halo {
+        int n = 0;
+        int* idx = nullptr;
+        Real* buf = nullptr;
+    };
+    std::array<Halo, 64> halo{};
+
+    rkdg_status halo_define(int slot, const int32_t* idx, int n) override {
+        if (slot < 0 || slot >= (int)halo.size() || n < 0 || (n > 0 && !idx))
+            return fail("bad halo slot", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (M.ens.n > 0) return fail("no halo exchange for ensembles", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (M.prm.slope_limiting || !M.levee_gp.empty())
+            return fail("no halo exchange with the slope limiter or levees", hipErrorInvalidValue), RKDG_ERR_ARG;
+        for (int i = 0; i < n; ++i)
+            if (idx[i] < 0 || idx[i] >= M.ne) return fail("halo element out of range", hipErrorInvalidValue), RKDG_ERR_ARG;
+        Halo& h = halo[slot];
+        if (h.idx) return fail("halo slot already defined", hipErrorInvalidValue), RKDG_ERR_ARG;
+        const std::vector<int> hi(idx, idx + n);
+        if (!upload(h.idx, hi) || !alloc(h.buf, (size_t)(3 * M.ndof + 1) * (n ? n : 1))) return RKDG_ERR_HIP;
+        h.n = n;
+        return RKDG_OK;
+    }
+    rkdg_status halo_buffer(int slot, void** buf, int64_t* bytes) override {
+        if (slot < 0 || slot >= (int)halo.size() || !halo[slot].idx)
+            return fail("halo slot not defined", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (buf) *buf = halo[slot].buf;
+        if (bytes) *bytes = (int64_t)(3 * M.ndof + 1) * halo[slot].n * (int64_t)sizeof(Real);
+        return RKDG_OK;
+    }
+    rkdg_status halo_move(int slot, bool pack, const void* src) override {
+        if (slot < 0 || slot >= (int)halo.size() || !halo[slot].idx)
+            return fail("halo slot not defined", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (!pack && !src) return fail("null halo buffer", hipErrorInvalidValue), RKDG_ERR_ARG;
+        const Halo& h = halo[slot];
+        Real* cur = q[M.stepper.stage];
+        Real* buf = pack ? h.buf : const_cast<Real*>(static_cast<const Real*>(src));
+        hipError_t e = rkdg::launch_halo<Real>(pack, M.ne, 3 * M.ndof, cur, wet[wcur], h.idx, h.n, buf, stream);
+        return e == hipSuccess ? RKDG_OK : (fail("halo kernel launch", e), RKDG_ERR_HIP);
+    }
+
     ~Engine() override {
         for (auto& pr : stage_ev) hipEventDestroy(pr.first), hipEventDestroy(pr.second);
         for (auto& pr : lim_ev) hipEventDestroy(pr.first), hipEventDestroy(pr.second);
@@ -806,6 +851,34 @@ rkdg_status rkdg_stage(rkdg_ctx* ctx) {
     rkdg_status s = ctx->eng->stage();
     if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
     return check_nan(ctx);
+}
+
+rkdg_status rkdg_halo_define(rkdg_ctx* ctx, int32_t slot, const int32_t* local_elements, int32_t n) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no halo exchange");
+    rkdg_status s = ctx->eng->halo_define(slot, local_elements, n);
+    return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
+}
+
+rkdg_status rkdg_halo_buffer(rkdg_ctx* ctx, int32_t slot, void** device_buffer, int64_t* bytes) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no halo exchange");
+    rkdg_status s = ctx->eng->halo_buffer(slot, device_buffer, bytes);
+    return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
+}
+
+rkdg_status rkdg_halo_pack(rkdg_ctx* ctx, int32_t slot) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no halo exchange");
+    rkdg_status s = ctx->eng->halo_move(slot, true, nullptr);
+    return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
+}
+
+rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buffer) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no halo exchange");
+    rkdg_status s = ctx->eng->halo_move(slot, false, device_buffer);
+    return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
 }
 
 rkdg_status rkdg_step(rkdg_ctx* ctx, int32_t nsteps) {

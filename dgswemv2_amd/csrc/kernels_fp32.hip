@@ -9,4 +9,5 @@ template hipError_t launch_limiter_prepare<float>(int, const LimiterArgs<float>&
 template hipError_t launch_limiter_limit<float>(int, const LimiterArgs<float>&, hipStream_t, int);
 template hipError_t launch_limiter_list<float>(int, const LimiterArgs<float>&, const int*, const int*, hipStream_t);
 template hipError_t launch_limiter_judge<float>(int, const LimiterArgs<float>&, const JudgeArgs&, hipStream_t, int);
+template hipError_t launch_halo<float>(bool, int, int, float*, uint8_t*, const int*, int, float*, hipStream_t);
 }  // namespace rkdg

@@ -9,6 +9,7 @@ template hipError_t launch_limiter_prepare<double>(int, const LimiterArgs<double
 template hipError_t launch_limiter_limit<double>(int, const LimiterArgs<double>&, hipStream_t, int);
 template hipError_t launch_limiter_list<double>(int, const LimiterArgs<double>&, const int*, const int*, hipStream_t);
 template hipError_t launch_limiter_judge<double>(int, const LimiterArgs<double>&, const JudgeArgs&, hipStream_t, int);
+template hipError_t launch_halo<double>(bool, int, int, double*, uint8_t*, const int*, int, double*, hipStream_t);
 }  // namespace rkdg
 
 namespace rkdg {

@@ -30,6 +30,12 @@ hipError_t launch_limiter_list(int p, const LimiterArgs<Real>& L, const int* lis
 template <typename Real>
 hipError_t launch_limiter_judge(int p, const LimiterArgs<Real>& L, const JudgeArgs& J, hipStream_t s, int block);
 
+// Halo exchange of a partitioned mesh: pack (state -> buffer) or unpack (buffer -> state) of n
+// listed elements; buffer [nrow + 1][n] (modes, then the wet flag).
+template <typename Real>
+hipError_t launch_halo(bool pack, int ne, int nrow, Real* q, uint8_t* wet, const int* idx, int n, Real* buf,
+                       hipStream_t s);
+
 // Learned indicator (indicator.hip): features (+ inference with mlp_on) over one state.
 template <typename Real>
 hipError_t launch_gate(int p, const GateArgs<Real>& G, bool mlp_on, hipStream_t s);

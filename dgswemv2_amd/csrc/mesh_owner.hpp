@@ -19,6 +19,7 @@ struct rkdg_mesh {
     std::vector<double> freq, ff, eq, amp, phase, manning;
     std::vector<int32_t> levee_nodes;
     std::vector<double> levee_param;
+    std::vector<int32_t> global_elt;  // submeshes: global element index of each local element
 
     void fill(rkdg_mesh_desc* d) const {
         d->n_nodes = (int32_t)(xyz.size() / 3);

@@ -1,0 +1,175 @@
+// Mesh partitioning for the distributed-boundary path: recursive coordinate bisection of the
+// elements and the submesh of one part (its elements plus ghost copies of the neighbours it
+// gathers from other parts).
+//
+// The reference partitions with METIS into submeshes whose cut edges become distributed
+// boundaries (partitioner/partition.cpp, preprocessor/mesh_metadata.cpp:139-223), and exchanges
+// edge traces per stage (rkdg_swe_proc_ompi_step.hpp:38-102).  Here a submesh keeps its ghost
+// elements as ordinary elements in global element order: an element of the part then sees exactly
+// the operands it sees in the whole mesh (neighbour modes, which side of an interface is the
+// lower ID, the order of its faces), so a partitioned run equals the whole-mesh run bit for bit
+// once the ghosts hold their owners' current state before each stage.  Faces of a ghost that
+// lead out of the submesh are typed land; ghosts' own results are overwritten by the exchange.
+#include <algorithm>
+#include <array>
+#include <map>
+#include <memory>
+#include <new>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/rkdg.h"
+#include "mesh_owner.hpp"
+
+namespace {
+
+struct PartError {
+    std::string msg;
+};
+
+uint64_t ekey(long a, long b) { return (uint64_t)std::min(a, b) << 32 | (uint64_t)std::max(a, b); }
+
+// Recursive coordinate bisection: split `idx` (element indices) across parts [p0, p0 + np) in
+// proportion to the part counts, along the longer extent of the barycentres (ties by index).
+void rcb(std::vector<int>& idx, size_t lo, size_t hi, int p0, int np, const std::vector<double>& cx,
+         const std::vector<double>& cy, int32_t* part) {
+    if (np == 1) {
+        for (size_t i = lo; i < hi; ++i) part[idx[i]] = p0;
+        return;
+    }
+    double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+    for (size_t i = lo; i < hi; ++i) {
+        x0 = std::min(x0, cx[idx[i]]), x1 = std::max(x1, cx[idx[i]]);
+        y0 = std::min(y0, cy[idx[i]]), y1 = std::max(y1, cy[idx[i]]);
+    }
+    const std::vector<double>& c = (x1 - x0 >= y1 - y0) ? cx : cy;
+    const int n1 = np / 2;
+    const size_t mid = lo + (size_t)((double)(hi - lo) * n1 / np + 0.5);
+    std::nth_element(idx.begin() + lo, idx.begin() + mid, idx.begin() + hi,
+                     [&](int a, int b) { return c[a] < c[b] || (c[a] == c[b] && a < b); });
+    rcb(idx, lo, mid, p0, n1, cx, cy, part);
+    rcb(idx, mid, hi, p0 + n1, np - n1, cx, cy, part);
+}
+
+}  // namespace
+
+extern "C" rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* d, int32_t nparts, int32_t* part) {
+    if (!d || !part || nparts < 1 || d->n_elements < nparts || !d->node_xyz || !d->elt_nodes) {
+        rkdg::set_create_error("bad arguments");
+        return RKDG_ERR_ARG;
+    }
+    const int ne = d->n_elements;
+    std::vector<double> cx(ne), cy(ne);
+    for (int e = 0; e < ne; ++e) {
+        double x = 0.0, y = 0.0;
+        for (int k = 0; k < 3; ++k) {
+            const int v = d->elt_nodes[3 * e + k];
+            if (v < 0 || v >= d->n_nodes) {
+                rkdg::set_create_error("element node out of range");
+                return RKDG_ERR_ARG;
+            }
+            x += d->node_xyz[3 * v], y += d->node_xyz[3 * v + 1];
+        }
+        cx[e] = x / 3.0, cy[e] = y / 3.0;
+    }
+    std::vector<int> idx(ne);
+    std::iota(idx.begin(), idx.end(), 0);
+    rcb(idx, 0, ne, 0, nparts, cx, cy, part);
+    return RKDG_OK;
+}
+
+extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part, int32_t p, rkdg_mesh** out,
+                                    rkdg_mesh_desc* sub, const int32_t** global_element) {
+    if (!d || !part || !out || !sub) {
+        rkdg::set_create_error("null argument");
+        return RKDG_ERR_ARG;
+    }
+    *out = nullptr;
+    try {
+        const long ne = d->n_elements, nn = d->n_nodes, nc = d->n_constituents;
+        if (d->n_levee_pairs > 0) throw PartError{"partitions of meshes with levees are not supported"};
+        std::unordered_map<uint64_t, std::array<long, 2>> owners;  // edge -> its elements (-1: none)
+        owners.reserve(3 * (size_t)ne);
+        for (long e = 0; e < ne; ++e)
+            for (int k = 0; k < 3; ++k) {
+                auto it = owners.emplace(ekey(d->elt_nodes[3 * e + (k + 1) % 3], d->elt_nodes[3 * e + (k + 2) % 3]),
+                                         std::array<long, 2>{e, -1});
+                if (!it.second) it.first->second[1] = e;
+            }
+        // local elements: the part's own and their neighbours, in global order
+        std::vector<char> local(ne, 0);
+        long nown = 0;
+        for (long e = 0; e < ne; ++e) {
+            if (part[e] != p) continue;
+            ++nown, local[e] = 1;
+            for (int k = 0; k < 3; ++k) {
+                const auto& o = owners[ekey(d->elt_nodes[3 * e + (k + 1) % 3], d->elt_nodes[3 * e + (k + 2) % 3])];
+                const long nb = o[0] == e ? o[1] : o[0];
+                if (nb >= 0) local[nb] = 1;
+            }
+        }
+        if (nown == 0) throw PartError{"part " + std::to_string(p) + " has no elements"};
+        std::unique_ptr<rkdg_mesh> M(new rkdg_mesh());
+        std::vector<long> nloc(nn, -1);
+        std::vector<char> nused(nn, 0);
+        for (long e = 0; e < ne; ++e)
+            if (local[e]) {
+                M->global_elt.push_back((int32_t)e);
+                for (int k = 0; k < 3; ++k) nused[d->elt_nodes[3 * e + k]] = 1;
+            }
+        long nl = 0;
+        for (long v = 0; v < nn; ++v)
+            if (nused[v]) {
+                nloc[v] = nl++;
+                for (int k = 0; k < 3; ++k) M->xyz.push_back(d->node_xyz[3 * v + k]);
+                if (d->node_manning_n) M->manning.push_back(d->node_manning_n[v]);
+            }
+        for (int32_t e : M->global_elt)
+            for (int k = 0; k < 3; ++k) M->elt.push_back((int32_t)nloc[d->elt_nodes[3 * e + k]]);
+        // boundary edges: the global ones among the local elements' faces, and land on ghost faces
+        // that lead out of the submesh
+        std::unordered_map<uint64_t, int> btype;
+        for (long i = 0; i < d->n_bound_edges; ++i)
+            btype[ekey(d->bound_edge_nodes[2 * i], d->bound_edge_nodes[2 * i + 1])] = d->bound_edge_type[i];
+        for (int32_t e : M->global_elt)
+            for (int k = 0; k < 3; ++k) {
+                const long a = d->elt_nodes[3 * e + (k + 1) % 3], b = d->elt_nodes[3 * e + (k + 2) % 3];
+                const auto& o = owners[ekey(a, b)];
+                const long nb = o[0] == e ? o[1] : o[0];
+                int t;
+                if (nb < 0) {
+                    auto it = btype.find(ekey(a, b));
+                    if (it == btype.end()) throw PartError{"boundary edge without a code"};
+                    t = it->second;
+                } else if (!local[nb]) {
+                    t = RKDG_BC_LAND;  // only ghosts have neighbours outside the submesh
+                } else {
+                    continue;
+                }
+                M->bedge.push_back((int32_t)nloc[a]), M->bedge.push_back((int32_t)nloc[b]);
+                M->btype.push_back((uint8_t)t);
+            }
+        if (nc > 0) {
+            M->freq.assign(d->frequency, d->frequency + nc);
+            M->ff.assign(d->forcing_fact, d->forcing_fact + nc);
+            M->eq.assign(d->equilib_arg, d->equilib_arg + nc);
+            for (long v = 0; v < nn; ++v)
+                if (nused[v])
+                    for (long c = 0; c < nc; ++c) {
+                        M->amp.push_back(d->node_amplitude ? d->node_amplitude[v * nc + c] : 0.0);
+                        M->phase.push_back(d->node_phase ? d->node_phase[v * nc + c] : 0.0);
+                    }
+        }
+        M->fill(sub);
+        if (global_element) *global_element = M->global_elt.data();
+        *out = M.release();
+        return RKDG_OK;
+    } catch (const PartError& err) {
+        rkdg::set_create_error(err.msg);
+    } catch (const std::bad_alloc&) {
+        rkdg::set_create_error("out of host memory");
+    }
+    return RKDG_ERR_ARG;
+}

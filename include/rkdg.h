@@ -299,6 +299,38 @@ rkdg_status rkdg_mesh_write_adcirc(const rkdg_mesh_desc* mesh, const char* name,
 rkdg_status rkdg_mesh_write_meta(const rkdg_mesh_desc* mesh, const char* name, const char* meta_path);
 rkdg_status rkdg_mesh_read_meta(const char* meta_path, const char* bcis_path, rkdg_mesh** out, rkdg_mesh_desc* desc);
 
+/* ---- partitions and halo exchange (the distributed-boundary path) ----------------------
+ * Replaces the reference's partitioned run (partitioner/partition.cpp; per-submesh .meta +
+ * .dbmd files, mesh_metadata.cpp:139-223; the per-stage exchange of distributed-boundary data,
+ * rkdg_swe_proc_ompi_step.hpp:38-102) for element-centric stages: a submesh holds its part's
+ * elements and ghost copies of the neighbouring elements of other parts, all in global element
+ * order, so each of its elements sees exactly the operands it sees in the whole mesh.  Before
+ * every stage the ghosts must hold their owners' current state: after rkdg_stage on every part,
+ * each part packs the elements other parts hold as ghosts (rkdg_halo_pack) and each part unpacks
+ * the buffers of its owners (rkdg_halo_unpack).  A partitioned run then equals the whole-mesh
+ * run bit for bit in every precision.  Not for levees or the slope limiter (both read neighbour
+ * data after the stage), nor ensembles. */
+
+/* Recursive coordinate bisection of the element barycentres into nparts parts of equal size
+ * (+-1), along the longer extent at each level: part[e] in [0, nparts). */
+rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* mesh, int32_t nparts, int32_t* part);
+/* Submesh of part p: its elements and their neighbours in other parts (ghosts), in global
+ * element order; nodes in global order; ghost faces leading out of the submesh are land.
+ * *global_element (may be NULL) receives the global index of each local element (owned by the
+ * returned mesh; free with rkdg_mesh_free). */
+rkdg_status rkdg_submesh(const rkdg_mesh_desc* mesh, const int32_t* part, int32_t p, rkdg_mesh** out,
+                         rkdg_mesh_desc* sub, const int32_t** global_element);
+/* Halo slots (up to 64 per context): slot s lists n local elements.  The slot's device buffer
+ * holds [3 ndof + 1][n] values of the context's precision (double for the FP64 forms, float for
+ * FP32): the modes of the current state and the wet flag (0/1).  pack: current state -> buffer;
+ * unpack: a buffer of the same layout (another context's slot, same precision and order) ->
+ * current state of the listed elements.  Both are stream-ordered on ctx's stream; order them
+ * across contexts with rkdg_sync. */
+rkdg_status rkdg_halo_define(rkdg_ctx* ctx, int32_t slot, const int32_t* local_elements, int32_t n);
+rkdg_status rkdg_halo_buffer(rkdg_ctx* ctx, int32_t slot, void** device_buffer, int64_t* bytes);
+rkdg_status rkdg_halo_pack(rkdg_ctx* ctx, int32_t slot);
+rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buffer);
+
 /* ---- output ----------------------------------------------------------------------------
  * The reference's writers, at a step boundary, single-mesh contexts.
  * rkdg_write_modal: write_modal_data (problem_postprocessor/swe_post_write_modal.hpp:5-90):

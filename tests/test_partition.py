@@ -1,0 +1,140 @@
+"""Partitions and the halo exchange (the distributed-boundary path).
+
+The reference's MPI run partitions the mesh (partitioner/partition.cpp) and exchanges
+distributed-boundary data every stage (rkdg_swe_proc_ompi_step.hpp:38-102); its serial and
+parallel runs are compared on L2 errors (scripts/correctness/compare_l2_errors.py:34-38).  Here a
+partitioned run must equal the whole-mesh run bit for bit: on the CPU with the oracle stepping
+each submesh and the ghosts refreshed from their owners after every stage, and on the GPU through
+rkdg_halo_pack / rkdg_halo_unpack in every precision.
+"""
+import numpy as np
+import pytest
+
+from dgswemv2_amd import (BC_LAND, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, PartitionedSimulation, RkdgError,
+                          Simulation, partition_rcb, submesh)
+from dgswemv2_amd import workloads as W
+from oracle.oracle_py import OracleSim
+
+
+def _adjacency(mesh):
+    edges = {}
+    for e, nodes in enumerate(mesh.elt_nodes.tolist()):
+        for k in range(3):
+            edges.setdefault(frozenset((nodes[(k + 1) % 3], nodes[(k + 2) % 3])), []).append(e)
+    nb = {e: set() for e in range(mesh.n_elements)}
+    for els in edges.values():
+        if len(els) == 2:
+            nb[els[0]].add(els[1]), nb[els[1]].add(els[0])
+    return edges, nb
+
+
+@pytest.mark.parametrize("nparts", [1, 3, 8])
+def test_partition_and_submeshes(nparts):
+    mesh, _ = W.tidal(scale=0.05)
+    part = partition_rcb(mesh, nparts)
+    counts = np.bincount(part, minlength=nparts)
+    assert counts.sum() == mesh.n_elements and counts.max() - counts.min() <= 1
+    edges, nb = _adjacency(mesh)
+    gbt = {frozenset(map(int, be)): int(t) for be, t in zip(mesh.bound_edges, mesh.bound_types)}
+    for p in range(nparts):
+        sub, gid = submesh(mesh, part, p)
+        own = set(np.flatnonzero(part == p).tolist())
+        ghosts = set().union(*(nb[e] for e in own)) - own
+        assert np.all(np.diff(gid) > 0) and set(gid.tolist()) == own | ghosts
+        # same triangles (through the submesh's node numbering), nodes in global order
+        xy = sub.node_xyz[sub.elt_nodes]
+        assert np.array_equal(xy, mesh.node_xyz[mesh.elt_nodes[gid]])
+        # boundary faces: the global ones keep their codes; ghost faces out of the submesh are land
+        sbt = {}
+        for be, t in zip(sub.bound_edges, sub.bound_types):
+            key = frozenset(tuple(sub.node_xyz[v, :2]) for v in be)
+            sbt[key] = int(t)
+        for le, e in enumerate(gid.tolist()):
+            nodes = mesh.elt_nodes[e].tolist()
+            for k in range(3):
+                ge = frozenset((nodes[(k + 1) % 3], nodes[(k + 2) % 3]))
+                key = frozenset(tuple(mesh.node_xyz[v, :2]) for v in ge)
+                if len(edges[ge]) == 1:
+                    assert sbt[key] == gbt[ge]
+                elif not set(edges[ge]) <= set(gid.tolist()):
+                    assert e in ghosts and sbt[key] == BC_LAND
+                else:
+                    assert key not in sbt
+
+
+def _oracle_partitioned(mesh, prm, nparts, stages):
+    """The oracle stepping each submesh, ghosts refreshed from their owners after every stage."""
+    part = partition_rcb(mesh, nparts)
+    subs = [submesh(mesh, part, p) for p in range(nparts)]
+    sims = [OracleSim(s, prm) for s, _ in subs]
+    for _ in range(stages):
+        for s in sims:
+            s.stage()
+        st = [s.state() for s in sims]
+        gq = np.zeros((3, st[0]["q"].shape[1], mesh.n_elements))
+        gw = np.zeros(mesh.n_elements, np.uint8)
+        for p, ((_, gid), S) in enumerate(zip(subs, st)):
+            own = part[gid] == p
+            gq[:, :, gid[own]] = S["q"][:, :, own]
+            gw[gid[own]] = S["wet"][own]
+        for (_, gid), s in zip(subs, sims):
+            s.set_state(np.ascontiguousarray(gq[:, :, gid]), np.ascontiguousarray(gw[gid]))
+    for s in sims:
+        s.close()
+    return gq, gw
+
+
+@pytest.mark.parametrize("name,kw,nparts,stages", [
+    ("tidal", dict(scale=0.04), 4, 9),
+    ("bowl", dict(scale=0.1, slope_limiting=False), 3, 12),
+    ("manufactured", dict(scale=0.2), 5, 6),
+])
+def test_oracle_partitioned_equals_whole(name, kw, nparts, stages):
+    mesh, prm = getattr(W, name)(**kw)
+    whole = OracleSim(mesh, prm)
+    for _ in range(stages):
+        whole.stage()
+    ref = whole.state()
+    whole.close()
+    q, wet = _oracle_partitioned(mesh, prm, nparts, stages)
+    assert np.array_equal(q, ref["q"]) and np.array_equal(wet, ref["wet"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [RKDG_FP64_PLAIN, RKDG_FP64_FAST, RKDG_FP32_BULK])
+@pytest.mark.parametrize("name,kw,nparts", [
+    ("tidal", dict(scale=0.05), 4),
+    ("bowl", dict(scale=0.1, slope_limiting=False), 3),
+    ("manufactured", dict(scale=0.25), 8),
+])
+def test_gpu_partitioned_equals_whole(precision, name, kw, nparts):
+    mesh, prm = getattr(W, name)(**kw)
+    a = Simulation(mesh, prm, precision)
+    b = PartitionedSimulation(mesh, prm, precision, nparts=nparts)
+    for k in range(3 * prm.nstages):
+        a.stage()
+        b.stage()
+        sa, sb = a.state(), b.state()
+        assert np.array_equal(sa["q"], sb["q"]) and np.array_equal(sa["wet"], sb["wet"]), k
+    a.close(), b.close()
+
+
+@pytest.mark.gpu
+def test_gpu_halo_errors():
+    mesh, prm = W.bowl(scale=0.05)  # slope limiter on: no halo exchange
+    s = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    with pytest.raises(RkdgError, match="slope limiter"):
+        s.halo_define(0, [0, 1])
+    s.close()
+    mesh, prm = W.tidal(scale=0.03)
+    s = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    with pytest.raises(RkdgError, match="out of range"):
+        s.halo_define(0, [0, mesh.n_elements])
+    with pytest.raises(RkdgError, match="not defined"):
+        s.halo_pack(5)
+    s.halo_define(1, [3, 4])
+    with pytest.raises(RkdgError, match="already defined"):
+        s.halo_define(1, [5])
+    buf, nbytes = s.halo_buffer(1)
+    assert buf and nbytes == 2 * (3 * 6 + 1) * 8
+    s.close()
