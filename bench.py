@@ -186,6 +186,48 @@ def learned_bench(steps, scale, warm=40, judge_steps=10):
                       "max_missed_change": js["max_missed_change"]}}
 
 
+def halo_bench(nparts, steps, warmup, scale):
+    """The tidal workload split into `nparts` parts (recursive coordinate bisection), each stepped
+    by its own context with the halo exchange (pack / unpack kernels) after every stage, in FP64
+    fast.  Reports the per-part stage kernel time, the exchange (pack + unpack, all parts) and the
+    halo sizes; the partitioned state is compared with the whole mesh's after the same steps."""
+    import numpy as np
+    from dgswemv2_amd import RKDG_FP64_FAST, PartitionedSimulation, Simulation, workloads as W
+    log("building the tidal mesh for", nparts, "parts, scale", scale)
+    mesh, prm = W.tidal(scale=scale)
+    ps = PartitionedSimulation(mesh, prm, RKDG_FP64_FAST, nparts=nparts)
+    ghosts = [int((~own).sum()) for own in ps.own]
+    ps.step(warmup)
+    for s in ps.sims:
+        s.reset_timers()
+    ns = prm.nstages
+    t_stage = t_exch = 0.0
+    for _ in range(steps * ns):
+        t0 = time.perf_counter()
+        for s in ps.sims:
+            s.stage()
+        for s in ps.sims:
+            s.sync()
+        t1 = time.perf_counter()
+        ps.exchange()
+        t_stage += t1 - t0
+        t_exch += time.perf_counter() - t1
+    reps = [s.report() for s in ps.sims]
+    q = ps.state()["q"]
+    ps.close()
+    ref = Simulation(mesh, prm, RKDG_FP64_FAST)
+    ref.step(warmup + steps)
+    same = bool(np.array_equal(ref.state()["q"], q))
+    ref.close()
+    nrow = 3 * 6 + 1
+    return {"parts": nparts, "triangles": mesh.n_elements, "form": "fp64_fast",
+            "elements_per_part": [int(own.sum()) for own in ps.own], "ghosts_per_part": ghosts,
+            "halo_bytes_per_stage": int(sum(ghosts) * nrow * 8),
+            "stage_kernel_ms_per_part": [r["stage_kernel_ms"] / max(1, r["stages_timed"]) for r in reps],
+            "stage_wall_ms": t_stage / (steps * ns) * 1e3, "exchange_wall_ms": t_exch / (steps * ns) * 1e3,
+            "equal_to_whole_mesh": same}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -201,9 +243,15 @@ def main():
                     help="time one form only (short profiled runs); the JSON line needs all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--halo-parts", type=int, default=0, metavar="N",
+                    help="only run the partitioned tidal workload (N parts, halo exchange) and exit")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.halo_parts:
+        r = halo_bench(args.halo_parts, args.steps, args.warmup, args.scale)
+        log("halo", json.dumps(r))
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine's multi-GPU sharding is "
