@@ -825,21 +825,33 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
     // ---- M^-1 and the SSP-RK combination (element.hpp:500-505, explicit_ssp_rk_stepper.hpp:67-80);
     // the first earlier state may wait in LDS (t0_lds, fetched when the stage started)
     if (t0_lds) lds_columns_wait();
+    // the first term's scalars are read once, outside the mode loop (a kernel-argument load
+    // inside it waits out its latency for every mode); further terms (schemes with more than two
+    // stages behind) take the general loop
+    const int nterm = A.nterm;
+    const Real ta0 = A.term_a[0], tc0 = term_coef<ENS>(A, IS, 0), ca = A.cur_a, cc = cur_coef<ENS>(A, IS);
+    const Real* const tq0 = A.term_q[0];
+    const Real* const ts0 = A.term_sol[0];
+    Real* const so = A.sol_out;
     Real nq[3][ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const Real sol = rhs[k][d] * (C::minv[0][d] / adet);
-            if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
+            if (so) stc(so, ne, k * ND + d, e, sol);
             Real acc = Real(0.0);
-            for (int i = 0; i < A.nterm; ++i) {
-                Real term = A.term_a[i] * ((i == 0 && t0_lds) ? t0_lds[64 * (k * ND + d) + lane_index()]
-                                                               : ldc(A.term_q[i], ne, k * ND + d, e));
-                if (A.term_sol[i]) term = term + term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
+            if (nterm > 0) {
+                Real term = ta0 * (t0_lds ? t0_lds[64 * (k * ND + d) + lane_index()] : ldc(tq0, ne, k * ND + d, e));
+                if (ts0) term = term + tc0 * ldc(ts0, ne, k * ND + d, e);
                 acc = acc + term;
+                for (int i = 1; i < nterm; ++i) {
+                    term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                    if (A.term_sol[i]) term = term + term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
+                    acc = acc + term;
+                }
             }
-            acc = acc + (A.cur_a * q[k][d] + cur_coef<ENS>(A, IS) * sol);
+            acc = acc + (ca * q[k][d] + cc * sol);
             nq[k][d] = acc;
         }
     finish_element<Real, P, WD>(A, e, wet, nq);
@@ -1325,20 +1337,30 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     }
 
     if (t0_lds) lds_columns_wait();
+    // the first term's scalars are read once, outside the mode loop (see stage_element)
+    const int nterm = A.nterm;
+    const Real ta0 = A.term_a[0], tc0 = term_coef<ENS>(A, IS, 0), ca = A.cur_a, cc = cur_coef<ENS>(A, IS);
+    const Real* const tq0 = A.term_q[0];
+    const Real* const ts0 = A.term_sol[0];
+    Real* const so = A.sol_out;
     Real nq[3][ND];
 #pragma unroll
     for (int k = 0; k < 3; ++k)
 #pragma unroll
         for (int d = 0; d < ND; ++d) {
             const Real sol = rhs[k][d] * C::minv[0][d];
-            if (A.sol_out) stc(A.sol_out, ne, k * ND + d, e, sol);
-            Real acc = A.cur_a * q[k][d] + cur_coef<ENS>(A, IS) * sol;
-            for (int i = 0; i < A.nterm; ++i) {
-                Real term = A.term_a[i] * ((kTerm0 && i == 0) ? t0[k][d]
-                                           : (i == 0 && t0_lds) ? t0_lds[64 * (k * ND + d) + lane_index()]
-                                                                : ldc(A.term_q[i], ne, k * ND + d, e));
-                if (A.term_sol[i]) term += term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
+            if (so) stc(so, ne, k * ND + d, e, sol);
+            Real acc = ca * q[k][d] + cc * sol;
+            if (nterm > 0) {
+                Real term = ta0 * (kTerm0 ? t0[k][d]
+                                   : t0_lds ? t0_lds[64 * (k * ND + d) + lane_index()] : ldc(tq0, ne, k * ND + d, e));
+                if (ts0) term += tc0 * ldc(ts0, ne, k * ND + d, e);
                 acc += term;
+                for (int i = 1; i < nterm; ++i) {
+                    term = A.term_a[i] * ldc(A.term_q[i], ne, k * ND + d, e);
+                    if (A.term_sol[i]) term += term_coef<ENS>(A, IS, i) * ldc(A.term_sol[i], ne, k * ND + d, e);
+                    acc += term;
+                }
             }
             nq[k][d] = acc;
         }
