@@ -293,7 +293,8 @@ class PartitionedSimulation:
 def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, problem=PROBLEM_NONE,
                 ic_constant=True, ze0=0.0, qx0=0.0, qy0=0.0, function_source=False, friction=FRICTION_NONE,
                 friction_cf=0.0, wetting_drying=False, h_o=0.1, slope_limiting=False, sl_M=1.0e-8, sl_nu=1.5,
-                problem_param=(0.0, 0.0, 0.0, 0.0)):
+                problem_param=(0.0, 0.0, 0.0, 0.0), spherical_mesh=False, sp_correction=False, sph_R=6378200.0,
+                sph_lon_o=0.0, sph_lat_o=0.0):
     """rkdg_params with the reference's input defaults (problem_input/swe_inputs.hpp:90-155)."""
     prm = N.Params()
     prm.p, prm.nstages, prm.order, prm.dt, prm.g, prm.ramp_duration = p, nstages, order, dt, g, ramp_duration
@@ -304,6 +305,8 @@ def make_params(p=1, nstages=2, order=2, dt=1.0, g=9.81, ramp_duration=0.0, prob
     prm.slope_limiting, prm.sl_M, prm.sl_nu = int(bool(slope_limiting)), sl_M, sl_nu
     for i in range(4):
         prm.problem_param[i] = problem_param[i]
+    prm.spherical_mesh, prm.sp_correction = int(bool(spherical_mesh)), int(bool(sp_correction))
+    prm.sph_R, prm.sph_lon_o, prm.sph_lat_o = sph_R, sph_lon_o, sph_lat_o
     return prm
 
 

@@ -51,6 +51,11 @@ class Params(C.Structure):
         ("wetting_drying", C.c_int32), ("h_o", C.c_double),
         ("slope_limiting", C.c_int32), ("sl_M", C.c_double), ("sl_nu", C.c_double),
         ("problem_param", C.c_double * 4),
+        ("spherical_mesh", C.c_int32),
+        ("sp_correction", C.c_int32),
+        ("sph_R", C.c_double),
+        ("sph_lon_o", C.c_double),
+        ("sph_lat_o", C.c_double),
     ]
 
 

@@ -73,6 +73,7 @@ struct Engine final : EngineBase {
     std::vector<Real*> q;      // reference state slots 0..nstages
     std::vector<Real*> sol;    // per stage, only where a later stage reads it
     Real *bath = nullptr, *geom = nullptr, *egeom = nullptr, *vxy = nullptr, *gmn2 = nullptr;
+    Real *sp_vol = nullptr, *sp_edge = nullptr;  // spherical correction (null: sp = 1)
     Real *famp = nullptr, *fphase = nullptr, *levee_gp = nullptr;
     int *nb = nullptr, *edge = nullptr, *eforce = nullptr, *neighbor = nullptr;
     uint32_t* meta = nullptr;
@@ -206,6 +207,7 @@ struct Engine final : EngineBase {
             if (needed && (!alloc(sol[s], S) || hipMemset(sol[s], 0, S * sizeof(Real)) != hipSuccess))
                 return RKDG_ERR_HIP;
         }
+        if (!M.sp_vol.empty() && (!upload(sp_vol, M.sp_vol) || !upload(sp_edge, M.sp_edge))) return RKDG_ERR_HIP;
         if (!upload(bath, M.bath) || !upload(geom, M.geom) || !upload(egeom, M.egeom) || !upload(vxy, M.vxy) ||
             !upload(gmn2, M.gmn2) || !upload(famp, M.famp) || !upload(fphase, M.fphase) ||
             !upload(levee_gp, M.levee_gp) || !upload(nb, M.nb) ||
@@ -422,6 +424,7 @@ struct Engine final : EngineBase {
         A.wet_in = wet[wcur], A.wet_out = wet[wcur ^ 1];
         A.bath_vrtx = bath_vrtx, A.gmn2 = gmn2, A.vxy = vxy, A.eforce = eforce, A.famp = famp, A.fphase = fphase;
         A.levee_gp = levee_gp;
+        A.sp_vol = sp_vol, A.sp_edge = sp_edge;
         A.ncon = M.ncon;
         for (int c = 0; c < M.ncon; ++c)
             A.freq[c] = (Real)M.freq[c], A.ffact[c] = (Real)M.ffact[c], A.eqarg[c] = (Real)M.eqarg[c];

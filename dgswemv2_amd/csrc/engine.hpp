@@ -71,6 +71,8 @@ struct StageArgs {
     const Real* vxy;         // [6][ne] vertex x0 x1 x2 y0 y1 y2
     const int* eforce;       // [nedge] tide/flow forcing record, levee record, or -1
     const Real* levee_gp;    // [nlevee][3][NE] barrier height, C_subcritical, C_supercritical at in-side points
+    const Real* sp_vol;      // [NG][ne] spherical correction at the volume points, or null (sp = 1; read
+    const Real* sp_edge;     // [NE][nedge] ... and at the edge points (in-side order)  only by CF kernels)
     const Real* famp;        // [nforced][ncon][NE]
     const Real* fphase;      // [nforced][ncon][NE]
     int ncon;
@@ -256,8 +258,9 @@ RKDG_HD inline void problem_source(double td, Real x, Real y, Real* s) {
 }
 
 // ---------------------------------------------------------------------------------------
-// Local Lax-Friedrichs flux (numerical_fluxes/rkdg_swe_LLF_flux.hpp:7-60) with sp = 1.
-// bath is the bathymetry of the side whose aux the caller passes.
+// Local Lax-Friedrichs flux (numerical_fluxes/rkdg_swe_LLF_flux.hpp:7-60).  bath and sp (the
+// spherical correction, 1 on a Cartesian mesh) are those of the side whose aux the caller passes;
+// sp scales the in-side flux's x-terms and the wave speed's x-part, not the ex-side flux (:45-57).
 
 template <typename Real>
 RKDG_HD inline Real rmax(Real a, Real b) {  // std::max semantics: (a < b) ? b : a
@@ -269,7 +272,8 @@ RKDG_HD inline Real rmin(Real a, Real b) {  // std::min semantics: (b < a) ? b :
 }
 
 template <typename Real>
-RKDG_HD inline void llf(Real grav, const Real* qi, const Real* qx, Real bath, Real nx, Real ny, Real* F) {
+RKDG_HD inline void llf(Real grav, const Real* qi, const Real* qx, Real bath, Real nx, Real ny, Real* F,
+                        Real sp = Real(1)) {
     using std::fabs;
     using std::sqrt;
     const Real h_in = qi[0] + bath;
@@ -280,15 +284,15 @@ RKDG_HD inline void llf(Real grav, const Real* qi, const Real* qx, Real bath, Re
     const Real v_ex = qx[2] / h_ex;
     const Real un_in = u_in * nx + v_in * ny;
     const Real un_ex = u_ex * nx + v_ex * ny;
-    const Real spc = nx * nx + ny * ny;
+    const Real spc = (nx * sp) * (nx * sp) + ny * ny;
     const Real lam = rmax(fabs(un_in) + sqrt(grav * h_in * spc), fabs(un_ex) + sqrt(grav * h_ex * spc));
     const Real uuh_in = u_in * qi[1], vvh_in = v_in * qi[2], uvh_in = u_in * qi[2];
     const Real pe_in = grav * ((qi[0] * qi[0]) / 2 + qi[0] * bath);
     const Real uuh_ex = u_ex * qx[1], vvh_ex = v_ex * qx[2], uvh_ex = u_ex * qx[2];
     const Real pe_ex = grav * ((qx[0] * qx[0]) / 2 + qx[0] * bath);
-    const Real fi0 = qi[1] * nx + qi[2] * ny;
-    const Real fi1 = (uuh_in + pe_in) * nx + uvh_in * ny;
-    const Real fi2 = uvh_in * nx + (vvh_in + pe_in) * ny;
+    const Real fi0 = sp * qi[1] * nx + qi[2] * ny;
+    const Real fi1 = sp * (uuh_in + pe_in) * nx + uvh_in * ny;
+    const Real fi2 = sp * uvh_in * nx + (vvh_in + pe_in) * ny;
     const Real fe0 = qx[1] * nx + qx[2] * ny;
     const Real fe1 = (uuh_ex + pe_ex) * nx + uvh_ex * ny;
     const Real fe2 = uvh_ex * nx + (vvh_ex + pe_ex) * ny;
@@ -299,12 +303,12 @@ RKDG_HD inline void llf(Real grav, const Real* qi, const Real* qx, Real bath, Re
 
 // Reflective (land) exterior state + LLF (boundary_conditions/rkdg_swe_bc_land.hpp:68-86).
 template <typename Real>
-RKDG_HD inline void land_llf(Real grav, Real nx, Real ny, const Real* qi, Real bath, Real* F) {
+RKDG_HD inline void land_llf(Real grav, Real nx, Real ny, const Real* qi, Real bath, Real* F, Real sp = Real(1)) {
     const Real tx = -ny, ty = nx;
     const Real qn_ex = -(qi[1] * nx + qi[2] * ny);
     const Real qt_ex = qi[1] * tx + qi[2] * ty;
     Real qe[3] = {qi[0], qn_ex * nx + qt_ex * tx, qn_ex * ny + qt_ex * ty};
-    llf(grav, qi, qe, bath, nx, ny, F);
+    llf(grav, qi, qe, bath, nx, ny, F, sp);
 }
 
 // Interface flux seen by one side (ISP::Internal::ComputeFlux, rkdg_swe_is_internal.hpp:24-94).
@@ -312,27 +316,27 @@ RKDG_HD inline void land_llf(Real grav, Real nx, Real ny, const Real* qi, Real b
 // bi/bx: their bathymetry.  Returns the flux of the requested side.
 template <typename Real, bool WD>
 RKDG_HD inline void interface_flux(Real grav, const Real* qi, const Real* qx, Real bi, Real bx, Real nx, Real ny,
-                                   bool wet_in, bool wet_ex, bool want_in, Real* F) {
+                                   bool wet_in, bool wet_ex, bool want_in, Real* F, Real sp = Real(1)) {
     Real Fi[3], Fe[3];
-    llf(grav, qi, qx, bi, nx, ny, Fi);
+    llf(grav, qi, qx, bi, nx, ny, Fi, sp);
     Fe[0] = -Fi[0], Fe[1] = -Fi[1], Fe[2] = -Fi[2];
     if (WD) {
         if (Fi[0] > Real(1e-12)) {
             if (!wet_in) {
                 Fi[0] = 0, Fi[1] = 0, Fi[2] = 0;
-                land_llf(grav, -nx, -ny, qx, bx, Fe);
+                land_llf(grav, -nx, -ny, qx, bx, Fe, sp);
             } else if (!wet_ex) {
                 const Real f0 = Fi[0];
-                llf(Real(0), qx, qi, bx, -nx, -ny, Fe);
+                llf(Real(0), qx, qi, bx, -nx, -ny, Fe, sp);
                 Fe[0] = -f0;
             }
         } else if (Fi[0] < Real(-1e-12)) {
             if (!wet_ex) {
                 Fe[0] = 0, Fe[1] = 0, Fe[2] = 0;
-                land_llf(grav, nx, ny, qi, bi, Fi);
+                land_llf(grav, nx, ny, qi, bi, Fi, sp);
             } else if (!wet_in) {
                 const Real fe0 = Fe[0];
-                llf(Real(0), qi, qx, bi, nx, ny, Fi);
+                llf(Real(0), qi, qx, bi, nx, ny, Fi, sp);
                 Fi[0] = -fe0;
             }
         }
@@ -352,7 +356,7 @@ RKDG_HD inline void interface_flux(Real grav, const Real* qi, const Real* qx, Re
 // gravity in its LLF flux.
 template <typename Real>
 RKDG_HD inline void levee_flux(Real grav, bool is_in, const Real* qo, const Real* qnb, Real bo, Real nx, Real ny,
-                               Real H, Real Cs, Real Cp, bool wet_own, Real* F) {
+                               Real H, Real Cs, Real Cp, bool wet_own, Real* F, Real sp = Real(1)) {
     using std::abs;
     using std::sqrt;
     const Real tol = Real(0.01);  // H_tolerance
@@ -380,7 +384,7 @@ RKDG_HD inline void levee_flux(Real grav, bool is_in, const Real* qo, const Real
         const bool receiving = from_in ? !is_in : is_in;
         if (receiving && !wet_own) gravity = Real(0.0);
     }
-    llf(gravity, qo, qe, bo, onx, ony, F);
+    llf(gravity, qo, qe, bo, onx, ony, F, sp);
 }
 
 // ---------------------------------------------------------------------------------------
@@ -586,8 +590,12 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
             const Real u = qg[1] / h, v = qg[2] / h;
             const Real uuh = u * qg[1], vvh = v * qg[2], uvh = u * qg[2];
             const Real pe = grav * (Real(0.5) * (qg[0] * qg[0]) + qg[0] * bg);
-            const Real fx[3] = {qg[1], uuh + pe, uvh};
+            Real fx[3] = {qg[1], uuh + pe, uvh};
             const Real fy[3] = {qg[2], uvh, vvh + pe};
+            if (CF && A.sp_vol) {  // Fx *= sp (rkdg_swe_proc_volume.hpp:25-30)
+                const Real sp = ldc(A.sp_vol, ne, g, e);
+                fx[0] = sp * fx[0], fx[1] = sp * fx[1], fx[2] = sp * fx[2];
+            }
 #pragma unroll
             for (int d = 0; d < ND; ++d) {
                 Real dx = Real(0.0);
@@ -755,22 +763,25 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
                     const Real qo_g[3] = {qo[0][g], qo[1][g], qo[2][g]};
                     const Real qn_g[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
                     Real Fg[3];
-                    levee_flux<Real>(grav, is_in, qo_g, qn_g, bo[g], nx, ny, lp[gi], lp[NE + gi], lp[2 * NE + gi], wet, Fg);
+                    const Real sp = (CF && A.sp_edge) ? ldc(A.sp_edge, A.nedge, gi, ed) : Real(1);
+                    levee_flux<Real>(grav, is_in, qo_g, qn_g, bo[g], nx, ny, lp[gi], lp[NE + gi], lp[2 * NE + gi], wet, Fg,
+                                     sp);
                     F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
                 }
             } else {
 #pragma unroll
                 for (int g = 0; g < NE; ++g) {
                     const int gm = NE - 1 - g;  // mirrored point on the other side
+                    const Real sp = (CF && A.sp_edge) ? ldc(A.sp_edge, A.nedge, is_in ? g : gm, ed) : Real(1);
                     Real Fg[3];
                     if (is_in) {
                         const Real qi[3] = {qo[0][g], qo[1][g], qo[2][g]};
                         const Real qx[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
-                        interface_flux<Real, WD>(grav, qi, qx, bo[g], bn[gm], nx, ny, wet, wet_nb, true, Fg);
+                        interface_flux<Real, WD>(grav, qi, qx, bo[g], bn[gm], nx, ny, wet, wet_nb, true, Fg, sp);
                     } else {
                         const Real qi[3] = {qn[0][gm], qn[1][gm], qn[2][gm]};
                         const Real qx[3] = {qo[0][g], qo[1][g], qo[2][g]};
-                        interface_flux<Real, WD>(grav, qi, qx, bn[gm], bo[g], nx, ny, wet_nb, wet, false, Fg);
+                        interface_flux<Real, WD>(grav, qi, qx, bn[gm], bo[g], nx, ny, wet_nb, wet, false, Fg, sp);
                     }
                     F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
                 }
@@ -803,7 +814,7 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
                     qe[0] = qi[0], qe[1] = qi[1], qe[2] = qi[2];  // unreachable: function BCs launch the CF build
                 }
                 Real Fg[3];
-                llf(grav, qi, qe, bo[g], nx, ny, Fg);
+                llf(grav, qi, qe, bo[g], nx, ny, Fg, (CF && A.sp_edge) ? ldc(A.sp_edge, A.nedge, g, ed) : Real(1));
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
         }
@@ -911,17 +922,22 @@ RKDG_HD inline double cbrt_rcpR(double x) {  // x^(-1/3): y <- y (4 - x y^3) / 3
     return y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
 }
 
-template <typename Real>
-RKDG_HD inline void llf_fast(Real grav, const Real* qi, const Real* qx, Real bath, Real nx, Real ny, Real* F) {
+// SPH: the spherical correction sp enters as in llf (wave speed sqrt(g h ((nx sp)^2 + ny^2)), the
+// in-side flux's x-terms scaled); otherwise sp = 1 and nx^2 + ny^2 = 1.
+template <typename Real, bool SPH = false>
+RKDG_HD inline void llf_fast(Real grav, const Real* qi, const Real* qx, Real bath, Real nx, Real ny, Real* F,
+                             Real sp = Real(1)) {
     const Real h_in = qi[0] + bath, h_ex = qx[0] + bath;
     const Real r_in = rcpR(h_in), r_ex = rcpR(h_ex);
     const Real u_in = qi[1] * r_in, v_in = qi[2] * r_in;
     const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
     const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
-    const Real lam = rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
+    const Real spc = SPH ? (nx * sp) * (nx * sp) + ny * ny : Real(1);
+    const Real lam = SPH ? rmax(absR(un_in) + sqrtR(grav * h_in * spc), absR(un_ex) + sqrtR(grav * h_ex * spc))
+                         : rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
     const Real pe_in = grav * qi[0] * (Real(0.5) * qi[0] + bath), pe_ex = grav * qx[0] * (Real(0.5) * qx[0] + bath);
-    const Real fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
-    const Real fi1 = u_in * fi0 + pe_in * nx, fe1 = u_ex * fe0 + pe_ex * nx;
+    const Real fi0 = (SPH ? sp * qi[1] : qi[1]) * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
+    const Real fi1 = u_in * fi0 + (SPH ? sp * pe_in : pe_in) * nx, fe1 = u_ex * fe0 + pe_ex * nx;
     const Real fi2 = v_in * fi0 + pe_in * ny, fe2 = v_ex * fe0 + pe_ex * ny;
     F[0] = Real(0.5) * (fi0 + fe0 + lam * (qi[0] - qx[0]));
     F[1] = Real(0.5) * (fi1 + fe1 + lam * (qi[1] - qx[1]));
@@ -931,18 +947,20 @@ RKDG_HD inline void llf_fast(Real grav, const Real* qi, const Real* qx, Real bat
 // The same LLF flux with the pressures of both states given relative to a hydrostatic
 // reference r: pi = pe_in - r, px = pe_ex - r.  The mass component and lambda are the plain
 // ones; the momentum components come out as F - r n.
-template <typename Real>
+template <typename Real, bool SPH = false>
 RKDG_HD inline void llf_rel(Real grav, const Real* qi, const Real* qx, Real bath, Real pi, Real px, Real nx,
-                            Real ny, Real* F) {
+                            Real ny, Real* F, Real sp = Real(1)) {
     const Real h_in = qi[0] + bath, h_ex = qx[0] + bath;
     const Real r_in = rcpR(h_in), r_ex = rcpR(h_ex);
     const Real u_in = qi[1] * r_in, v_in = qi[2] * r_in;
     const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
     const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
-    const Real lam = rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
-    const Real fi0 = qi[1] * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
+    const Real spc = SPH ? (nx * sp) * (nx * sp) + ny * ny : Real(1);
+    const Real lam = SPH ? rmax(absR(un_in) + sqrtR(grav * h_in * spc), absR(un_ex) + sqrtR(grav * h_ex * spc))
+                         : rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
+    const Real fi0 = (SPH ? sp * qi[1] : qi[1]) * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
     F[0] = Real(0.5) * (fi0 + fe0 + lam * (qi[0] - qx[0]));
-    F[1] = Real(0.5) * (u_in * fi0 + u_ex * fe0 + (pi + px) * nx + lam * (qi[1] - qx[1]));
+    F[1] = Real(0.5) * (u_in * fi0 + u_ex * fe0 + ((SPH ? sp * pi : pi) + px) * nx + lam * (qi[1] - qx[1]));
     F[2] = Real(0.5) * (v_in * fi0 + v_ex * fe0 + (pi + px) * ny + lam * (qi[2] - qx[2]));
 }
 
@@ -950,12 +968,12 @@ RKDG_HD inline void llf_rel(Real grav, const Real* qi, const Real* qx, Real bath
 // caller's hydrostatic reference at this point.  Returns the caller's flux (w.r.t. its outward
 // normal) with momentum relative to r.  The wet/dry fixes of rkdg_swe_is_internal.hpp:47-90
 // (rare) are applied to the plain flux, rebuilt as F_rel + r n.
-template <typename Real, bool WD>
+template <typename Real, bool WD, bool SPH = false>
 RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* qx, Real pi, Real px, Real bi,
                                         Real bx, Real nx, Real ny, bool wet_in, bool wet_ex, bool want_in, Real r,
-                                        Real* F) {
+                                        Real* F, Real sp = Real(1)) {
     Real Fi[3];
-    llf_rel(grav, qi, qx, bi, pi, px, nx, ny, Fi);
+    llf_rel<Real, SPH>(grav, qi, qx, bi, pi, px, nx, ny, Fi, sp);
     if (WD && ((Fi[0] > Real(1e-12) && !(wet_in && wet_ex)) || (Fi[0] < -Real(1e-12) && !(wet_in && wet_ex)))) {
         Fi[1] += r * nx, Fi[2] += r * ny;
         Real Fe[3] = {-Fi[0], -Fi[1], -Fi[2]};
@@ -965,10 +983,10 @@ RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* q
                 const Real tx = ny, ty = -nx, mx = -nx, my = -ny;
                 const Real qn_ex = -(qx[1] * mx + qx[2] * my), qt_ex = qx[1] * tx + qx[2] * ty;
                 const Real qe[3] = {qx[0], qn_ex * mx + qt_ex * tx, qn_ex * my + qt_ex * ty};
-                llf_fast(grav, qx, qe, bx, mx, my, Fe);
+                llf_fast<Real, SPH>(grav, qx, qe, bx, mx, my, Fe, sp);
             } else {  // !wet_ex
                 const Real f0 = Fi[0];
-                llf_fast(Real(0.0), qx, qi, bx, -nx, -ny, Fe);
+                llf_fast<Real, SPH>(Real(0.0), qx, qi, bx, -nx, -ny, Fe, sp);
                 Fe[0] = -f0;
             }
         } else {
@@ -977,10 +995,10 @@ RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* q
                 const Real tx = -ny, ty = nx;
                 const Real qn_ex = -(qi[1] * nx + qi[2] * ny), qt_ex = qi[1] * tx + qi[2] * ty;
                 const Real qe[3] = {qi[0], qn_ex * nx + qt_ex * tx, qn_ex * ny + qt_ex * ty};
-                llf_fast(grav, qi, qe, bi, nx, ny, Fi);
+                llf_fast<Real, SPH>(grav, qi, qe, bi, nx, ny, Fi, sp);
             } else {  // !wet_in
                 const Real fe0 = Fe[0];
-                llf_fast(Real(0.0), qi, qx, bi, nx, ny, Fi);
+                llf_fast<Real, SPH>(Real(0.0), qi, qx, bi, nx, ny, Fi, sp);
                 Fi[0] = -fe0;
             }
         }
@@ -1095,7 +1113,12 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     // large canceling terms (g ze b with b up to kilometres) never form in FP32.
     constexpr Real PHI0 = C::phi[0][0];  // the constant mode's value
     const Real zc = q[0][0] * PHI0;      // mean surface elevation (mode 0)
-    const Real zr = wet ? zc : Real(0.0);     // dry elements keep the plain form (no volume term)
+    // With the spherical correction the volume x-fluxes carry sp and the source does not, so the
+    // hydrostatic part no longer cancels against a face integral: spherical runs keep the plain
+    // pressures (reference 0), as dry elements do.
+    const bool sph = CF && A.sp_vol != nullptr;
+    const Real zr = (wet && !sph) ? zc : Real(0.0);
+    const Real zv = sph ? zc : Real(0.0);  // added to the volume's elevation relative to zc
 
     // ---- volume and source: rhs (unscaled by |det J|) = sum_g [idphi . (J^-T F) + ipf . S]
     Real rhs[3][ND];
@@ -1118,14 +1141,19 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
             }
             const Real h = (zc + bg) + zp, rh = rcpR(h);
             const Real u = q1 * rh, v = q2 * rh;
-            const Real pe = grav * zp * (h - Real(0.5) * zp);  // relative to g zc (zc/2 + b)
+            const Real zpv = zv + zp;  // elevation relative to the volume's reference (zc, or 0)
+            const Real pe = grav * zpv * (h - Real(0.5) * zpv);  // relative to g zc (zc/2 + b), or plain
             const Real uvh = u * q2;
-            const Real fx[3] = {q1, u * q1 + pe, uvh};
+            Real fx[3] = {q1, u * q1 + pe, uvh};
             const Real fy[3] = {q2, uvh, v * q2 + pe};
+            if (sph) {  // Fx *= sp (rkdg_swe_proc_volume.hpp:25-30)
+                const Real sp = ldc(A.sp_vol, ne, g, e);
+                fx[0] *= sp, fx[1] *= sp, fx[2] *= sp;
+            }
             Real G1[3], G2[3];
 #pragma unroll
             for (int k = 0; k < 3; ++k) G1[k] = J00 * fx[k] + J01 * fy[k], G2[k] = J10 * fx[k] + J11 * fy[k];
-            const Real gz = grav * zp;
+            const Real gz = grav * (zv + zp);
             Real S[3] = {Real(0.0), gz * (bz1 * J00 + bz2 * J10), gz * (bz1 * J01 + bz2 * J11)};
             if (CF && A.fsrc) {
                 const Real x = A.vxy[e] * C::psi[0][g] + A.vxy[(size_t)ne + e] * C::psi[1][g] +
@@ -1191,18 +1219,21 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
                 }
                 const Real own[3] = {zc + zo, o1, o2}, nbs[3] = {zcn + zn, n1, n2};
                 // surface elevations relative to zr, and the pressures relative to g zr (zr/2 + bo)
-                const Real zpo = wet ? zo : own[0];
+                const Real zpo = (wet && !sph) ? zo : own[0];
                 const Real zpn = dzr + zn;
                 const Real B = is_in ? bo : bn;        // the LLF uses the in-side bathymetry
                 const Real D = is_in ? Real(0.0) : zr * (bn - bo);
                 const Real po = grav * (zpo * (zr + Real(0.5) * zpo + B) + D);
                 const Real pnb = grav * (zpn * (zr + Real(0.5) * zpn + B) + D);
                 const Real rref = grav * zr * (Real(0.5) * zr + bo);
+                const Real sp = sph ? ldc(A.sp_edge, A.nedge, is_in ? g : gm, ed[f]) : Real(1);
                 Real Fg[3];
                 if (is_in)
-                    interface_flux_fast<Real, WD>(grav, own, nbs, po, pnb, bo, bn, fnx, fny, wet, wet_nb[f], true, rref, Fg);
+                    interface_flux_fast<Real, WD, CF>(grav, own, nbs, po, pnb, bo, bn, fnx, fny, wet, wet_nb[f], true, rref, Fg,
+                                                      sp);
                 else
-                    interface_flux_fast<Real, WD>(grav, nbs, own, pnb, po, bn, bo, fnx, fny, wet_nb[f], wet, false, rref, Fg);
+                    interface_flux_fast<Real, WD, CF>(grav, nbs, own, pnb, po, bn, bo, fnx, fny, wet_nb[f], wet, false, rref,
+                                                      Fg, sp);
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
             const Real scale = fsJ * rdet;
@@ -1264,7 +1295,8 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
                 }
                 const Real own[3] = {zc + zo, o1, o2}, nbs[3] = {zn, Real(0), Real(0)};
                 Real Fg[3];
-                levee_flux<Real>(grav, is_in, own, nbs, bo, fnx, fny, lp[gi], lp[NE + gi], lp[2 * NE + gi], wet, Fg);
+                levee_flux<Real>(grav, is_in, own, nbs, bo, fnx, fny, lp[gi], lp[NE + gi], lp[2 * NE + gi], wet, Fg,
+                                 sph ? ldc(A.sp_edge, A.nedge, gi, fed) : Real(1));
                 const Real rref = grav * zr * (Real(0.5) * zr + bo);
                 F[0][g] = Fg[0], F[1][g] = Fg[1] - rref * onx, F[2][g] = Fg[2] - rref * ony;
             }
@@ -1306,19 +1338,21 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
                     o1 = fmaR(q[1][d], w, o1), o2 = fmaR(q[2][d], w, o2), bo = fmaR(bath[d], w, bo);
                 }
                 const Real qi[3] = {zc + zo, o1, o2};
-                Real qe[3], zpe = zo;  // exterior elevation relative to zc (land, flow: the interior one)
+                // elevations relative to the reference zb: zc, or 0 (plain pressures) when spherical
+                const Real zb = sph ? Real(0.0) : zc, zob = sph ? qi[0] : zo;
+                Real qe[3], zpe = zob;  // exterior elevation (land, flow: the interior one)
                 if (bc == 0) {
                     const Real tx = -fny, ty = fnx;
                     const Real qn_ex = -(qi[1] * fnx + qi[2] * fny), qt_ex = qi[1] * tx + qi[2] * ty;
                     qe[0] = qi[0], qe[1] = qn_ex * fnx + qt_ex * tx, qe[2] = qn_ex * fny + qt_ex * ty;
                 } else {
                     exterior_state<Real, P, CF>(A, bc, e, fed, f, g, qi, fnx, fny, qe, IS.t, IS.ramp);
-                    if (bc != 2) zpe = qe[0] - zc;  // tide, function: the exterior elevation is given
+                    if (bc != 2) zpe = qe[0] - zb;  // tide, function: the exterior elevation is given
                 }
-                const Real po = grav * zo * (zc + Real(0.5) * zo + bo);
-                const Real pe = grav * zpe * (zc + Real(0.5) * zpe + bo);
+                const Real po = grav * zob * (zb + Real(0.5) * zob + bo);
+                const Real pe = grav * zpe * (zb + Real(0.5) * zpe + bo);
                 Real Fg[3];
-                llf_rel(grav, qi, qe, bo, po, pe, fnx, fny, Fg);
+                llf_rel<Real, CF>(grav, qi, qe, bo, po, pe, fnx, fny, Fg, sph ? ldc(A.sp_edge, A.nedge, g, fed) : Real(1));
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
             const Real scale = fsJ * rdet;

@@ -210,6 +210,22 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
                     return "levee partner edges do not pair up";
     }
 
+    // ---- spherical meshes: (longitude, latitude) in degrees projected onto a cylinder before
+    // anything else (preprocess_mesh_data, swe_pre_preprocess_mesh.hpp:5-27)
+    std::vector<double> proj;
+    const double* xyz = md.node_xyz;
+    if (prm.spherical_mesh) {
+        if (!(prm.sph_R > 0)) return "spherical mesh needs a positive radius";
+        const double R = prm.sph_R, R_o = R * std::cos(prm.sph_lat_o * kPI / 180.0);
+        proj.assign(md.node_xyz, md.node_xyz + 3 * (size_t)md.n_nodes);
+        for (int v = 0; v < md.n_nodes; ++v) {
+            proj[3 * (size_t)v] = R_o * (proj[3 * (size_t)v] - prm.sph_lon_o) * kPI / 180.0;
+            proj[3 * (size_t)v + 1] = R * proj[3 * (size_t)v + 1] * kPI / 180.0;
+        }
+        xyz = proj.data();
+    }
+    if (prm.sp_correction && !(prm.sph_R > 0)) return "spherical correction needs a positive radius";
+
     // ---- element geometry (Element::Initialize, element.hpp:166-272)
     std::vector<Tri> tri(ne);
     M.geom.assign(5 * (size_t)ne, 0.0);
@@ -218,7 +234,7 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
         Tri& T = tri[e];
         for (int k = 0; k < 3; ++k) {
             const int v = md.elt_nodes[3 * e + k];
-            T.x[k] = md.node_xyz[3 * v], T.y[k] = md.node_xyz[3 * v + 1], T.z[k] = md.node_xyz[3 * v + 2];
+            T.x[k] = xyz[3 * v], T.y[k] = xyz[3 * v + 1], T.z[k] = xyz[3 * v + 2];
         }
         if (T.det() < 0) {  // coordinates-only CCW swap (shape_straighttriangle.cpp:9-11)
             std::swap(T.x[0], T.x[2]);
@@ -321,6 +337,33 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
     M.nedge = (int)eg[0].size();
     M.closed_forms = prm.function_source != 0;
     for (auto& b : bedges) M.closed_forms |= (b[0] == RKDG_BC_FUNCTION);
+
+    // ---- spherical correction sp = cos(lat_o) / cos(y / R) at the volume points and at each
+    // edge's in-side points (swe_pre_init_data.hpp:31-48, 80-160), y interpolated from the
+    // element's (CCW-fixed) vertices with the linear shape functions, (y0 psi0 + y1 psi1) + y2 psi2.
+    // Built into the kernels with the closed-form problem functions.
+    if (prm.sp_correction) {
+        const double cphi = std::cos(prm.sph_lat_o * kPI / 180.0), R = prm.sph_R;
+        auto y_at = [&](int e, double z1, double z2) {
+            const Tri& T = tri[e];
+            return (T.y[0] * (-(z1 + z2) / 2) + T.y[1] * ((1 + z1) / 2)) + T.y[2] * ((1 + z2) / 2);
+        };
+        M.sp_vol.assign((size_t)ng * ne, 1.0);
+        for (int e = 0; e < ne; ++e)
+            for (int g = 0; g < ng; ++g)
+                M.sp_vol[(size_t)g * ne + e] = cphi / std::cos(y_at(e, M.master.rule.z[g][0], M.master.rule.z[g][1]) / R);
+        M.sp_edge.assign((size_t)nge * M.nedge, 1.0);
+        for (int e = 0; e < ne; ++e)
+            for (int k = 0; k < 3; ++k) {
+                const int id = face_edge[3 * e + k];
+                const bool in_side = nbr[3 * e + k] < 0 || nbr[3 * e + k] > e;  // boundaries: the element
+                if (id < 0 || !in_side) continue;
+                for (int g = 0; g < nge; ++g)
+                    M.sp_edge[(size_t)g * M.nedge + id] =
+                        cphi / std::cos(y_at(e, M.master.zedge[k][g][0], M.master.zedge[k][g][1]) / R);
+            }
+        M.closed_forms = true;
+    }
     M.egeom.resize(3 * (size_t)M.nedge);
     for (int c = 0; c < 3; ++c) std::copy(eg[c].begin(), eg[c].end(), M.egeom.begin() + (size_t)c * M.nedge);
 
@@ -550,6 +593,7 @@ std::string merge_models(const std::vector<const Model*>& parts, Model& out) {
             b.sl_M != a.sl_M || b.sl_nu != a.sl_nu || x.problem != y.problem || x.function_source != y.function_source ||
             b.reference_face_order != a.reference_face_order)
             return "ensemble instances need equal physics options (g, friction, wetting-drying, limiter, problem)";
+        if (!b.sp_vol.empty()) return "no spherical correction in ensembles";
         if (b.ncon != a.ncon || b.freq != a.freq || b.ffact != a.ffact || b.eqarg != a.eqarg)
             return "ensemble instances need one harmonic constituent list";
         for (int i = 0; i < 4; ++i)

@@ -55,6 +55,8 @@ struct Model {
 
     // per edge record: interfaces (in-side geometry) then boundaries (own geometry)
     std::vector<double> egeom;      // [3][nedge] nx ny sJ
+    std::vector<double> sp_vol;     // [ngp][ne] spherical correction sp at the volume points (empty: sp = 1)
+    std::vector<double> sp_edge;    // [ngpe][nedge] sp at the edge points, in-side order (empty: sp = 1)
     std::vector<int> eforce;        // [nedge] tide/flow forcing record, levee record, or -1
     std::vector<double> levee_gp;   // [nlevee][3][ngpe] H_barrier, C_subcritical, C_supercritical at in-side points
     int ncon = 0;

@@ -119,6 +119,17 @@ typedef struct rkdg_params {
     double sl_M;
     double sl_nu;
     double problem_param[4]; /* problem-specific constants (dam break: x0, h_left, h_right) */
+    /* Spherical coordinates (SWE::SphericalProjection, swe_inputs.hpp:10-15).  spherical_mesh:
+     * node x / y are longitude / latitude in degrees, projected onto a cylinder before anything
+     * else, x = R cos(lat_o) (lon - lon_o) pi / 180, y = R lat pi / 180
+     * (swe_pre_preprocess_mesh.hpp:5-27).  sp_correction: the x-flux correction factor
+     * sp = cos(lat_o) / cos(y / R) at every Gauss point (swe_pre_init_data.hpp:31-48, 80-160),
+     * applied to the volume x-fluxes and in the LLF flux (rkdg_swe_LLF_flux.hpp:14-45); otherwise
+     * sp = 1.  Zero-initialised parameters keep both off; the reference's defaults are R =
+     * 6378200 m, lon_o = lat_o = 0. */
+    int32_t spherical_mesh;
+    int32_t sp_correction;
+    double sph_R, sph_lon_o, sph_lat_o;
 } rkdg_params;
 
 typedef struct rkdg_perf_report {

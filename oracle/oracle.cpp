@@ -735,6 +735,11 @@ static Sim* create(const rkdg_mesh_desc* md, const rkdg_params* prm) {
             unsigned nd = E.node_ID[k];
             E.shape.x[k] = md->node_xyz[3 * nd], E.shape.y[k] = md->node_xyz[3 * nd + 1];
             E.shape.z[k] = md->node_xyz[3 * nd + 2];
+            if (prm->spherical_mesh) {  // preprocess_mesh_data (swe_pre_preprocess_mesh.hpp:5-27)
+                const double R = prm->sph_R, R_o = R * std::cos(prm->sph_lat_o * PI / 180.0);
+                E.shape.x[k] = R_o * (E.shape.x[k] - prm->sph_lon_o) * PI / 180.0;
+                E.shape.y[k] = R * E.shape.y[k] * PI / 180.0;
+            }
         }
         if (E.shape.detJ() < 0) {  // CCW fix swaps coordinates only (shape_straighttriangle.cpp:9-11)
             std::swap(E.shape.x[0], E.shape.x[2]);
@@ -890,7 +895,11 @@ static Sim* create(const rkdg_mesh_desc* md, const rkdg_params* prm) {
             Mat db = mul(E.bath, E.dphi_gp[dir]);
             for (unsigned g = 0; g < ngp; ++g) E.db_at_gp(dir, g) = db(0, g);
         }
-        for (unsigned g = 0; g < ngp; ++g) E.aux_at_gp(2, g) = 1.0;  // no spherical projection
+        // spherical correction sp = cos(lat_o) / cos(y_gp / R) (swe_pre_init_data.hpp:31-48), else 1
+        for (unsigned g = 0; g < ngp; ++g)
+            E.aux_at_gp(2, g) = prm->sp_correction
+                                    ? std::cos(prm->sph_lat_o * PI / 180.0) / std::cos(E.gp_global[g][1] / prm->sph_R)
+                                    : 1.0;
         if (prm->ic_constant) {
             Mat un(3, 3);
             for (int k = 0; k < 3; ++k) un(0, k) = prm->ze0, un(1, k) = prm->qx0, un(2, k) = prm->qy0;
@@ -910,6 +919,19 @@ static Sim* create(const rkdg_mesh_desc* md, const rkdg_params* prm) {
     };
     for (auto& I : S->intfaces) edge_bath(I.in), edge_bath(I.ex);
     for (auto& B : S->bounds) edge_bath(B.s);
+    if (prm->sp_correction) {  // at the in-side points, mirrored on the ex side (:80-160)
+        const double cphi = std::cos(prm->sph_lat_o * PI / 180.0), R = prm->sph_R;
+        for (auto& I : S->intfaces) {
+            const int ngp = (int)I.in.gp_global.size();
+            for (int g = 0; g < ngp; ++g) {
+                const double sp = cphi / std::cos(I.in.gp_global[g][1] / R);
+                I.in.aux_at_gp(2, g) = sp, I.ex.aux_at_gp(2, ngp - 1 - g) = sp;
+            }
+        }
+        for (auto& B : S->bounds)
+            for (size_t g = 0; g < B.s.gp_global.size(); ++g)
+                B.s.aux_at_gp(2, (int)g) = cphi / std::cos(B.s.gp_global[g][1] / R);
+    }
     if (prm->friction == RKDG_FRICTION_MANNING) {  // :166-197
         for (auto& E : S->elts) {
             double nn[3];
