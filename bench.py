@@ -243,6 +243,8 @@ def main():
                     help="time one form only (short profiled runs); the JSON line needs all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--order", choices=("file", "shuffled", "hilbert"), default="file",
+                    help="tidal mesh element order: as generated, random, or random then Hilbert-renumbered")
     ap.add_argument("--halo-parts", type=int, default=0, metavar="N",
                     help="only run the partitioned tidal workload (N parts, halo exchange) and exit")
     args = ap.parse_args()
@@ -263,6 +265,12 @@ def main():
 
     log("building the tidal mesh, scale", args.scale)
     mesh, prm = W.tidal(scale=args.scale)
+    if args.order != "file":  # element order of the mesh file: random, or random then renumbered
+        from dgswemv2_amd import renumber
+        mesh, _ = W.shuffled(mesh)
+        if args.order == "hilbert":
+            mesh, _ = renumber(mesh)
+        log("element order", args.order)
     ne, ndof, ns = mesh.n_elements, 6, prm.nstages
     log("triangles", ne, "dt", prm.dt)
 
@@ -341,7 +349,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"tidal p=2 SSP-RK3, {ne} triangles (708x708 quads, jitter 20%, Fourier depth "
                                f"5-4000 m, M2 tide, Manning 0.025)", "triangles": ne, "p": 2, "stages": ns,
-                   "form": head},
+                   "form": head, "element_order": args.order},
         "forms": forms,
         "fp32_over_fp64_plain": forms["fp32_bulk"]["value"] / forms["fp64_plain"]["value"],
         "fp64_fast_over_fp64_plain": forms["fp64_fast"]["value"] / forms["fp64_plain"]["value"],

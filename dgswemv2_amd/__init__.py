@@ -17,7 +17,7 @@ from ._native import (BC_FLOW, BC_FUNCTION, BC_LAND, BC_LEVEE, BC_TIDE, FRICTION
                       PROBLEM_PARABOLIC_BOWL, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError)
 
 __all__ = [
-    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "read_meta", "write_meta", "partition_rcb", "submesh", "PartitionedSimulation", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
+    "Mesh", "quad_mesh", "read_adcirc", "write_adcirc", "read_meta", "write_meta", "partition_rcb", "submesh", "renumber", "PartitionedSimulation", "make_params", "Simulation", "Batch", "MlpTrainer", "RkdgError", "ndof_of",
     "BC_LAND", "BC_TIDE", "BC_FLOW", "BC_FUNCTION", "BC_LEVEE",
     "PROBLEM_NONE", "PROBLEM_MANUFACTURED", "PROBLEM_PARABOLIC_BOWL", "PROBLEM_DAM_BREAK",
     "FRICTION_NONE", "FRICTION_CHEZY", "FRICTION_MANNING", "RKDG_FP64_PLAIN", "RKDG_FP32_BULK", "RKDG_FP64_FAST",
@@ -209,6 +209,20 @@ def submesh(mesh, part, p):
         raise RkdgError(st, lib.rkdg_create_error().decode())
     try:
         return _mesh_from_desc(desc), np.ctypeslib.as_array(gid, shape=(desc.n_elements,)).copy()
+    finally:
+        lib.rkdg_mesh_free(handle)
+
+
+def renumber(mesh):
+    """The mesh with elements in Hilbert-curve order of their barycentres (nodes in order of first
+    use) and, per new element, its index in `mesh`."""
+    lib = N.load_library()
+    handle, desc, old = C.c_void_p(), N.MeshDesc(), C.POINTER(C.c_int32)()
+    st = lib.rkdg_mesh_renumber(C.byref(mesh.desc), C.byref(handle), C.byref(desc), C.byref(old))
+    if st != N.RKDG_OK:
+        raise RkdgError(st, lib.rkdg_create_error().decode())
+    try:
+        return _mesh_from_desc(desc), np.ctypeslib.as_array(old, shape=(desc.n_elements,)).copy()
     finally:
         lib.rkdg_mesh_free(handle)
 

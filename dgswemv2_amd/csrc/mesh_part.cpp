@@ -173,3 +173,105 @@ extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part
     }
     return RKDG_ERR_ARG;
 }
+
+// ---- locality renumbering ----------------------------------------------------------------
+// Elements in the order of a Hilbert curve through their barycentres (2^16 x 2^16 grid over the
+// mesh's bounding box, ties by index); nodes in order of first use by the renumbered elements.
+// Neighbours then sit close in memory, so the stage kernel's neighbour gathers hit in L2 on meshes
+// whose file order is arbitrary.  The reference's partitioner plays this role for its submeshes.
+
+namespace {
+
+uint64_t hilbert_d(uint32_t n, uint32_t x, uint32_t y) {  // position along the curve of order log2(n)
+    uint64_t d = 0;
+    for (uint32_t s = n / 2; s > 0; s /= 2) {
+        const uint32_t rx = (x & s) > 0, ry = (y & s) > 0;
+        d += (uint64_t)s * s * ((3 * rx) ^ ry);
+        if (ry == 0) {  // rotate the quadrant
+            if (rx == 1) x = s - 1 - x, y = s - 1 - y;
+            std::swap(x, y);
+        }
+    }
+    return d;
+}
+
+}  // namespace
+
+extern "C" rkdg_status rkdg_mesh_renumber(const rkdg_mesh_desc* d, rkdg_mesh** out, rkdg_mesh_desc* sub,
+                                          const int32_t** old_element) {
+    if (!d || !out || !sub || !d->node_xyz || !d->elt_nodes || d->n_elements <= 0) {
+        rkdg::set_create_error("bad arguments");
+        return RKDG_ERR_ARG;
+    }
+    *out = nullptr;
+    try {
+        const long ne = d->n_elements, nn = d->n_nodes, nc = d->n_constituents;
+        std::vector<double> cx(ne), cy(ne);
+        double x0 = 1e300, x1 = -1e300, y0 = 1e300, y1 = -1e300;
+        for (long e = 0; e < ne; ++e) {
+            double x = 0.0, y = 0.0;
+            for (int k = 0; k < 3; ++k) {
+                const int v = d->elt_nodes[3 * e + k];
+                if (v < 0 || v >= nn) throw PartError{"element node out of range"};
+                x += d->node_xyz[3 * v], y += d->node_xyz[3 * v + 1];
+            }
+            cx[e] = x / 3.0, cy[e] = y / 3.0;
+            x0 = std::min(x0, cx[e]), x1 = std::max(x1, cx[e]), y0 = std::min(y0, cy[e]), y1 = std::max(y1, cy[e]);
+        }
+        const uint32_t n = 1u << 16;
+        const double sx = x1 > x0 ? (n - 1) / (x1 - x0) : 0.0, sy = y1 > y0 ? (n - 1) / (y1 - y0) : 0.0;
+        std::vector<std::pair<uint64_t, long>> key(ne);
+        for (long e = 0; e < ne; ++e)
+            key[e] = {hilbert_d(n, (uint32_t)((cx[e] - x0) * sx), (uint32_t)((cy[e] - y0) * sy)), e};
+        std::sort(key.begin(), key.end());
+        std::unique_ptr<rkdg_mesh> M(new rkdg_mesh());
+        std::vector<long> nnew(nn, -1);
+        long nl = 0;
+        for (const auto& kv : key) {
+            const long e = kv.second;
+            M->global_elt.push_back((int32_t)e);
+            for (int k = 0; k < 3; ++k) {
+                const int v = d->elt_nodes[3 * e + k];
+                if (nnew[v] < 0) nnew[v] = nl++;
+                M->elt.push_back((int32_t)nnew[v]);
+            }
+        }
+        for (long v = 0; v < nn; ++v)
+            if (nnew[v] < 0) nnew[v] = nl++;  // nodes no element uses keep a place at the end
+        std::vector<long> nold(nn);
+        for (long v = 0; v < nn; ++v) nold[nnew[v]] = v;
+        for (long i = 0; i < nn; ++i) {
+            const long v = nold[i];
+            for (int k = 0; k < 3; ++k) M->xyz.push_back(d->node_xyz[3 * v + k]);
+            if (d->node_manning_n) M->manning.push_back(d->node_manning_n[v]);
+            for (long c = 0; c < nc; ++c) {
+                M->amp.push_back(d->node_amplitude ? d->node_amplitude[v * nc + c] : 0.0);
+                M->phase.push_back(d->node_phase ? d->node_phase[v * nc + c] : 0.0);
+            }
+        }
+        if (nc > 0) {
+            M->freq.assign(d->frequency, d->frequency + nc);
+            M->ff.assign(d->forcing_fact, d->forcing_fact + nc);
+            M->eq.assign(d->equilib_arg, d->equilib_arg + nc);
+        }
+        for (long i = 0; i < d->n_bound_edges; ++i) {
+            M->bedge.push_back((int32_t)nnew[d->bound_edge_nodes[2 * i]]);
+            M->bedge.push_back((int32_t)nnew[d->bound_edge_nodes[2 * i + 1]]);
+            M->btype.push_back(d->bound_edge_type[i]);
+        }
+        for (long i = 0; i < d->n_levee_pairs; ++i) {
+            M->levee_nodes.push_back((int32_t)nnew[d->levee_nodes[2 * i]]);
+            M->levee_nodes.push_back((int32_t)nnew[d->levee_nodes[2 * i + 1]]);
+            for (int k = 0; k < 3; ++k) M->levee_param.push_back(d->levee_param[3 * i + k]);
+        }
+        M->fill(sub);
+        if (old_element) *old_element = M->global_elt.data();
+        *out = M.release();
+        return RKDG_OK;
+    } catch (const PartError& err) {
+        rkdg::set_create_error(err.msg);
+    } catch (const std::bad_alloc&) {
+        rkdg::set_create_error("out of host memory");
+    }
+    return RKDG_ERR_ARG;
+}

@@ -20,7 +20,7 @@ import math
 
 import numpy as np
 
-from . import (BC_FUNCTION, BC_LAND, BC_TIDE, FRICTION_MANNING, PROBLEM_DAM_BREAK, PROBLEM_MANUFACTURED,
+from . import (BC_FUNCTION, BC_LAND, BC_TIDE, FRICTION_MANNING, PROBLEM_DAM_BREAK, PROBLEM_MANUFACTURED, Mesh,
                PROBLEM_NONE, PROBLEM_PARABOLIC_BOWL, make_params, quad_mesh)
 
 M2_OMEGA = 1.405189e-4
@@ -101,3 +101,25 @@ def ensemble(n=64, scale=1.0, seed=4, ne_min=20000, ne_max=400000, nstages=3, or
 
 
 WORKLOADS = {"manufactured": manufactured, "bowl": bowl, "tidal": tidal, "dambreak": dambreak}
+
+
+def shuffled(mesh, seed=1):
+    """The same mesh with its elements and nodes in random order (a mesh file with no locality),
+    and, per new element, its index in `mesh`."""
+    rng = np.random.default_rng(seed)
+    pe = rng.permutation(mesh.n_elements)
+    pn = rng.permutation(len(mesh.node_xyz))  # new -> old
+    inv = np.empty_like(pn)
+    inv[pn] = np.arange(pn.size)
+    kw = {}
+    if len(mesh.frequency):
+        nc = len(mesh.frequency)
+        kw = dict(frequency=mesh.frequency, forcing_fact=mesh.forcing_fact, equilib_arg=mesh.equilib_arg,
+                  node_amplitude=mesh.node_amplitude.reshape(-1, nc)[pn].reshape(-1),
+                  node_phase=mesh.node_phase.reshape(-1, nc)[pn].reshape(-1))
+    if mesh.node_manning_n is not None:
+        kw["node_manning_n"] = mesh.node_manning_n[pn]
+    if len(mesh.levee_nodes):
+        kw["levee_nodes"], kw["levee_param"] = inv[mesh.levee_nodes], mesh.levee_param
+    return Mesh(mesh.node_xyz[pn], inv[mesh.elt_nodes[pe]], inv[mesh.bound_edges], mesh.bound_types, **kw), pe
+

@@ -331,6 +331,15 @@ rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* mesh, int32_t nparts, int32
  * returned mesh; free with rkdg_mesh_free). */
 rkdg_status rkdg_submesh(const rkdg_mesh_desc* mesh, const int32_t* part, int32_t p, rkdg_mesh** out,
                          rkdg_mesh_desc* sub, const int32_t** global_element);
+/* Locality renumbering: elements in the order of a Hilbert curve through their barycentres, nodes
+ * in order of first use (unused nodes last), every per-node and per-edge array carried along, so
+ * neighbouring elements sit close in memory (the stage kernel's neighbour gathers then hit in L2
+ * on meshes with an arbitrary file order).  *old_element (may be NULL) receives, per new element,
+ * its index in `mesh` (owned by the returned mesh).  The renumbered mesh is the same
+ * discretisation; results differ from the original order only by rounding (which side of an
+ * interface is the lower ID, and face order, follow the numbering). */
+rkdg_status rkdg_mesh_renumber(const rkdg_mesh_desc* mesh, rkdg_mesh** out, rkdg_mesh_desc* renumbered,
+                               const int32_t** old_element);
 /* Halo slots (up to 64 per context): slot s lists n local elements.  The slot's device buffer
  * holds [3 ndof + 1][n] values of the context's precision (double for the FP64 forms, float for
  * FP32): the modes of the current state and the wet flag (0/1).  pack: current state -> buffer;

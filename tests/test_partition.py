@@ -138,3 +138,47 @@ def test_gpu_halo_errors():
     buf, nbytes = s.halo_buffer(1)
     assert buf and nbytes == 2 * (3 * 6 + 1) * 8
     s.close()
+
+
+# ---- locality renumbering (rkdg_mesh_renumber)
+def _close(mesh, window=256):
+    """Share of neighbour pairs less than `window` elements apart (one work-group's tile)."""
+    _, nb = _adjacency(mesh)
+    d = np.array([abs(e - n) for e, s in nb.items() for n in s])
+    return float(np.mean(d < window))
+
+
+def test_renumber_locality_and_same_mesh():
+    from dgswemv2_amd import renumber
+    mesh, prm = W.tidal(scale=0.05)
+    sh, pe = W.shuffled(mesh)
+    rn, old = renumber(sh)
+    assert sorted(old.tolist()) == list(range(mesh.n_elements))
+    # the same triangles (by coordinates), and neighbours close in index again
+    assert np.array_equal(rn.node_xyz[rn.elt_nodes], sh.node_xyz[sh.elt_nodes[old]])
+    assert _close(rn) > 0.85 and _close(sh) < 0.25, (_close(rn), _close(sh))
+    # the same discretisation: the oracle's runs agree to rounding, element by element
+    a = OracleSim(mesh, prm)
+    b = OracleSim(rn, prm)
+    a.step(4), b.step(4)
+    qa, qb = a.state()["q"], b.state()["q"]
+    qa_in_rn = qa[:, :, pe[old]]
+    for k in range(3):
+        assert np.linalg.norm(qb[k] - qa_in_rn[k]) <= 1e-12 * np.linalg.norm(qa[k]), k
+    a.close(), b.close()
+
+
+def test_renumber_weir_levees():
+    from dgswemv2_amd import renumber
+    from test_levee import weir_mesh, weir_params
+    mesh = weir_mesh()
+    rn, old = renumber(mesh)
+    prm = weir_params(p=1, slope_limiting=False)
+    prm.ze0 = 0.5
+    a, b = OracleSim(mesh, prm), OracleSim(rn, prm)
+    a.step(5), b.step(5)
+    qa, qb = a.state()["q"][:, :, old], b.state()["q"]
+    assert np.abs(qa[1:]).max() > 0
+    # rounding-level differences against the size of the state (the momenta are small here)
+    assert np.linalg.norm(qb - qa) <= 1e-13 * np.linalg.norm(qa)
+    a.close(), b.close()
