@@ -121,8 +121,10 @@ def ensemble_bench(steps, warmup, scale):
 def learned_bench(steps, scale, warm=40, judge_steps=10):
     """Learned troubled-cell indicator on the 500k-triangle dam break (p = 1, SSP-RK2, wetting-drying,
     Cockburn-Shu limiter).  FP32 bulk with the plain limiter runs `warm` steps with capture on
-    (features and labels of every limiter stage), the MLP is trained on the GPU, and then `steps`
-    steps are timed from the warm state in FP64 plain, FP32 plain and FP32 with the learned gate.
+    (features and labels of every limiter stage), the MLP is trained on the GPU and refined by one
+    aggregation round (the warm-up replayed under the learned gate, labelled by the judge), and
+    then `steps` steps are timed from the warm state in FP64 plain, FP32 plain and FP32 with the
+    learned gate.
     A further `judge_steps` window with the judge on (the plain limiter's decision on the learned
     run's states) gives the miss rate; L2 errors are taken against FP64 plain after the same steps."""
     import numpy as np
@@ -142,8 +144,14 @@ def learned_bench(steps, scale, warm=40, judge_steps=10):
     t0 = time.perf_counter()
     w, thr, info = IND.train(X, y, epochs=6, batch=1024, pos_weight=8.0, recall=0.9999, margin=16.0)
     train_s = time.perf_counter() - t0
-    del X, y
     log("trained in", round(train_s, 2), "s:", json.dumps(info))
+    # one aggregation round over the same warm-up window, under the learned gate
+    t0 = time.perf_counter()
+    w, thr, info, X, y = IND.refine(lambda: Simulation(mesh, prm, RKDG_FP32_BULK), warm, w, thr, X, y, epochs=3,
+                                    batch=1024, pos_weight=8.0, recall=0.9999, margin=16.0)
+    refine_s = time.perf_counter() - t0
+    del X, y
+    log("refined in", round(refine_s, 2), "s:", json.dumps(info))
     b = Simulation(mesh, prm, RKDG_FP32_BULK)
     b.step(warm)
     b.set_indicator(w, threshold=thr)
@@ -176,8 +184,9 @@ def learned_bench(steps, scale, warm=40, judge_steps=10):
             "timed_steps": steps, "dtype": "f32 (FP64 limiter arithmetic), bf16 MFMA gate", **res,
             "learned_over_fp64_plain": res["fp32_learned"]["value"] / res["fp64_plain"]["value"],
             "learned_over_fp32_plain": res["fp32_learned"]["value"] / res["fp32_plain"]["value"],
-            "training": {"samples": info["samples"], "positives": info["positives"], "epochs": 6,
-                         "collect_s": collect_s, "train_s": train_s, "val_recall": info["val_recall"],
+            "training": {"samples": info["samples"], "positives": info["positives"], "epochs": 6, "refine_epochs": 3,
+                         "collect_s": collect_s, "train_s": train_s, "refine_s": refine_s,
+                         "refine_samples": info["refine_samples"], "val_recall": info["val_recall"],
                          "val_open": info["val_open"], "final_loss": info["epoch_loss"][-1]},
             "judge": {"steps": judge_steps, "candidates": js["candidates"], "gate_open": js["flagged"],
                       "significant_write_backs": js["significant"],
