@@ -279,8 +279,15 @@ class PartitionedSimulation:
             s.sync()
 
     def stage(self):
-        for s in self.sims:
-            s.stage()
+        if self.params.slope_limiting:  # the limiter reads the neighbours' new state
+            for s in self.sims:
+                s.stage_begin()
+            self.exchange()
+            for s in self.sims:
+                s.stage_finish()
+        else:
+            for s in self.sims:
+                s.stage()
         self.exchange()
 
     def step(self, nsteps=1):
@@ -367,6 +374,13 @@ class Simulation:
 
     def sync(self):
         self._check(self.lib.rkdg_sync(self.ctx))
+
+    def stage_begin(self):
+        """The stage kernel of one stage (rkdg_stage_begin); stage_finish runs its limiter."""
+        self._check(self.lib.rkdg_stage_begin(self.ctx))
+
+    def stage_finish(self):
+        self._check(self.lib.rkdg_stage_finish(self.ctx))
 
     # halo exchange of a partitioned mesh (rkdg_halo_*)
     def halo_define(self, slot, local_elements):

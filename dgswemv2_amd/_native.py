@@ -143,6 +143,8 @@ SIGNATURES = {
                                C.POINTER(C.POINTER(C.c_int32))]),
     "rkdg_mesh_renumber": (C.c_int, [C.POINTER(MeshDesc), C.POINTER(C.c_void_p), C.POINTER(MeshDesc),
                                      C.POINTER(C.POINTER(C.c_int32))]),
+    "rkdg_stage_begin": (C.c_int, [C.c_void_p]),
+    "rkdg_stage_finish": (C.c_int, [C.c_void_p]),
     "rkdg_halo_define": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     "rkdg_halo_buffer": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "rkdg_halo_pack": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -37,6 +37,12 @@ struct EngineBase {
     virtual ~EngineBase() = default;
     virtual rkdg_status init() = 0;
     virtual rkdg_status stage() = 0;
+    // the stage in two parts (a partitioned run exchanges its halo between them): the stage
+    // kernel (with the stepper advance), then the limiter (and an ensemble's stepper advance)
+    virtual rkdg_status stage_begin() = 0;
+    virtual rkdg_status stage_finish() = 0;
+    bool mid_stage = false;      // between stage_begin and stage_finish
+
     virtual rkdg_status set_state(const double* q, const uint8_t* wet) = 0;
     virtual rkdg_status get_state(double* q, uint8_t* wet, uint8_t* limited, uint8_t* troubled) = 0;
     virtual rkdg_status nan_key(unsigned long long* key) = 0;
@@ -134,8 +140,7 @@ struct Engine final : EngineBase {
         if (slot < 0 || slot >= (int)halo.size() || n < 0 || (n > 0 && !idx))
             return fail("bad halo slot", hipErrorInvalidValue), RKDG_ERR_ARG;
         if (M.ens.n > 0) return fail("no halo exchange for ensembles", hipErrorInvalidValue), RKDG_ERR_ARG;
-        if (M.prm.slope_limiting || !M.levee_gp.empty())
-            return fail("no halo exchange with the slope limiter or levees", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (!M.levee_gp.empty()) return fail("no halo exchange with levees", hipErrorInvalidValue), RKDG_ERR_ARG;
         for (int i = 0; i < n; ++i)
             if (idx[i] < 0 || idx[i] >= M.ne) return fail("halo element out of range", hipErrorInvalidValue), RKDG_ERR_ARG;
         Halo& h = halo[slot];
@@ -398,6 +403,14 @@ struct Engine final : EngineBase {
     }
 
     rkdg_status stage() override {
+        rkdg_status r = stage_begin();
+        return r == RKDG_OK ? stage_finish() : r;
+    }
+    unsigned long long mid_key = 0;  // stage_key of the stage begun
+    bool mid_tm = false;
+
+    rkdg_status stage_begin() override {
+        if (mid_stage) return fail("stage_begin twice without stage_finish", hipErrorInvalidValue), RKDG_ERR_ARG;
         rkdg::Stepper& st = M.stepper;
         const int s = st.stage, ns = st.nstages;
         rkdg::StageArgs<Real> A;
@@ -467,6 +480,19 @@ struct Engine final : EngineBase {
             }
         }
 
+        mid_key = key, mid_tm = tm, mid_stage = true;
+        return RKDG_OK;
+    }
+
+    rkdg_status stage_finish() override {
+        if (!mid_stage) return fail("stage_finish without stage_begin", hipErrorInvalidValue), RKDG_ERR_ARG;
+        mid_stage = false;
+        rkdg::Stepper& st = M.stepper;
+        const int ns = st.nstages;
+        const bool ens = M.ens.n > 0;
+        const unsigned long long key = mid_key;
+        const bool tm = mid_tm;
+        hipError_t e = hipSuccess;
         if (M.prm.slope_limiting) {
             rkdg::LimiterArgs<Real> L;
             std::memset(&L, 0, sizeof L);
@@ -852,6 +878,21 @@ rkdg_status rkdg_stage(rkdg_ctx* ctx) {
     if (!ctx) return RKDG_ERR_ARG;
     if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
     rkdg_status s = ctx->eng->stage();
+    if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
+    return check_nan(ctx);
+}
+
+rkdg_status rkdg_stage_begin(rkdg_ctx* ctx) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
+    rkdg_status s = ctx->eng->stage_begin();
+    return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
+}
+
+rkdg_status rkdg_stage_finish(rkdg_ctx* ctx) {
+    if (!ctx) return RKDG_ERR_ARG;
+    if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: use the rkdg_batch_* calls");
+    rkdg_status s = ctx->eng->stage_finish();
     if (s != RKDG_OK) return set_err(ctx, s, ctx->eng->err);
     return check_nan(ctx);
 }

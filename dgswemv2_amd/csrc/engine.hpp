@@ -922,6 +922,12 @@ RKDG_HD inline double cbrt_rcpR(double x) {  // x^(-1/3): y <- y (4 - x y^3) / 3
     return y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
 }
 
+// Depth under the wave speed's square root.  FP32: a depth the FP64 wetting-drying pass left at
+// or just above zero at a vertex can come back from the FP32 modes a rounding error below zero at
+// a trace point; it is taken as zero there (the square root would give NaN).  FP64: unchanged.
+RKDG_HD inline float depth_for_speed(float h) { return h > 0.0f ? h : 0.0f; }
+RKDG_HD inline double depth_for_speed(double h) { return h; }
+
 // SPH: the spherical correction sp enters as in llf (wave speed sqrt(g h ((nx sp)^2 + ny^2)), the
 // in-side flux's x-terms scaled); otherwise sp = 1 and nx^2 + ny^2 = 1.
 template <typename Real, bool SPH = false>
@@ -933,8 +939,9 @@ RKDG_HD inline void llf_fast(Real grav, const Real* qi, const Real* qx, Real bat
     const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
     const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
     const Real spc = SPH ? (nx * sp) * (nx * sp) + ny * ny : Real(1);
-    const Real lam = SPH ? rmax(absR(un_in) + sqrtR(grav * h_in * spc), absR(un_ex) + sqrtR(grav * h_ex * spc))
-                         : rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
+    const Real ci = grav * depth_for_speed(h_in), cx = grav * depth_for_speed(h_ex);
+    const Real lam = SPH ? rmax(absR(un_in) + sqrtR(ci * spc), absR(un_ex) + sqrtR(cx * spc))
+                         : rmax(absR(un_in) + sqrtR(ci), absR(un_ex) + sqrtR(cx));
     const Real pe_in = grav * qi[0] * (Real(0.5) * qi[0] + bath), pe_ex = grav * qx[0] * (Real(0.5) * qx[0] + bath);
     const Real fi0 = (SPH ? sp * qi[1] : qi[1]) * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
     const Real fi1 = u_in * fi0 + (SPH ? sp * pe_in : pe_in) * nx, fe1 = u_ex * fe0 + pe_ex * nx;
@@ -956,8 +963,9 @@ RKDG_HD inline void llf_rel(Real grav, const Real* qi, const Real* qx, Real bath
     const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
     const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
     const Real spc = SPH ? (nx * sp) * (nx * sp) + ny * ny : Real(1);
-    const Real lam = SPH ? rmax(absR(un_in) + sqrtR(grav * h_in * spc), absR(un_ex) + sqrtR(grav * h_ex * spc))
-                         : rmax(absR(un_in) + sqrtR(grav * h_in), absR(un_ex) + sqrtR(grav * h_ex));
+    const Real ci = grav * depth_for_speed(h_in), cx = grav * depth_for_speed(h_ex);
+    const Real lam = SPH ? rmax(absR(un_in) + sqrtR(ci * spc), absR(un_ex) + sqrtR(cx * spc))
+                         : rmax(absR(un_in) + sqrtR(ci), absR(un_ex) + sqrtR(cx));
     const Real fi0 = (SPH ? sp * qi[1] : qi[1]) * nx + qi[2] * ny, fe0 = qx[1] * nx + qx[2] * ny;
     F[0] = Real(0.5) * (fi0 + fe0 + lam * (qi[0] - qx[0]));
     F[1] = Real(0.5) * (u_in * fi0 + u_ex * fe0 + ((SPH ? sp * pi : pi) + px) * nx + lam * (qi[1] - qx[1]));

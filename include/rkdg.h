@@ -319,9 +319,18 @@ rkdg_status rkdg_mesh_read_meta(const char* meta_path, const char* bcis_path, rk
  * every stage the ghosts must hold their owners' current state: after rkdg_stage on every part,
  * each part packs the elements other parts hold as ghosts (rkdg_halo_pack) and each part unpacks
  * the buffers of its owners (rkdg_halo_unpack).  A partitioned run then equals the whole-mesh
- * run bit for bit in every precision.  Not for levees or the slope limiter (both read neighbour
- * data after the stage), nor ensembles. */
+ * run bit for bit in every precision.  With the slope limiter the exchange also runs between the
+ * stage kernel and the limiter (rkdg_stage_begin / rkdg_stage_finish); the state then still equals
+ * the whole-mesh run's, while the `troubled` output flags of elements near a cut may differ (the
+ * trouble check's dilation does not cross parts; RKDG never reads those flags).  Not for levees
+ * across parts, nor ensembles. */
 
+/* One stage in two halves, for a partitioned run with the slope limiter: rkdg_stage_begin runs
+ * the stage kernel (the state then is the new stage's, before limiting), rkdg_stage_finish the
+ * limiter (with its NaN scan); rkdg_stage is both.  A partitioned run exchanges its halo after
+ * each half. */
+rkdg_status rkdg_stage_begin(rkdg_ctx* ctx);
+rkdg_status rkdg_stage_finish(rkdg_ctx* ctx);
 /* Recursive coordinate bisection of the element barycentres into nparts parts of equal size
  * (+-1), along the longer extent at each level: part[e] in [0, nparts). */
 rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* mesh, int32_t nparts, int32_t* part);

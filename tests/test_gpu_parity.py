@@ -262,3 +262,14 @@ def test_weir_levee_fast_forms_vs_oracle(p):
     banner("weir fast forms p", p, "fp64-fast", e64, "fp32", e32)
     assert max(e64) <= TOL64
     assert max(e32) <= 1e-5
+
+
+def test_fp32_coarse_bowl_no_nan():
+    """Regression: on the coarse bowl a depth the FP64 wetting-drying pass leaves at ~0 came back
+    from the FP32 modes slightly negative at a trace point and the LLF wave speed's square root
+    gave NaN in the first stage.  The FP32 wave speed takes such a depth as zero."""
+    mesh, prm = W.bowl(scale=0.1)
+    s = Simulation(mesh, prm, RKDG_FP32_BULK)
+    s.step(30)
+    assert np.all(np.isfinite(s.state()["q"]))
+    s.close()

@@ -106,6 +106,8 @@ def test_oracle_partitioned_equals_whole(name, kw, nparts, stages):
     ("tidal", dict(scale=0.05), 4),
     ("bowl", dict(scale=0.1, slope_limiting=False), 3),
     ("manufactured", dict(scale=0.25), 8),
+    ("bowl", dict(scale=0.1), 4),         # wetting-drying + slope limiter
+    ("dambreak", dict(scale=0.06), 5),    # wetting-drying + slope limiter, a moving front
 ])
 def test_gpu_partitioned_equals_whole(precision, name, kw, nparts):
     mesh, prm = getattr(W, name)(**kw)
@@ -121,10 +123,16 @@ def test_gpu_partitioned_equals_whole(precision, name, kw, nparts):
 
 @pytest.mark.gpu
 def test_gpu_halo_errors():
-    mesh, prm = W.bowl(scale=0.05)  # slope limiter on: no halo exchange
-    s = Simulation(mesh, prm, RKDG_FP64_PLAIN)
-    with pytest.raises(RkdgError, match="slope limiter"):
+    from test_levee import weir_mesh, weir_params
+    s = Simulation(weir_mesh(), weir_params(), RKDG_FP64_PLAIN)  # levees: no halo exchange
+    with pytest.raises(RkdgError, match="levees"):
         s.halo_define(0, [0, 1])
+    with pytest.raises(RkdgError, match="without stage_begin"):
+        s.stage_finish()
+    s.stage_begin()
+    with pytest.raises(RkdgError, match="twice"):
+        s.stage_begin()
+    s.stage_finish()
     s.close()
     mesh, prm = W.tidal(scale=0.03)
     s = Simulation(mesh, prm, RKDG_FP64_PLAIN)
