@@ -274,7 +274,7 @@ class PartitionedSimulation:
         for s in self.sims:
             s.sync()
         for p, q in self.links:
-            self.sims[q].halo_unpack(32 + p, self.sims[p].halo_buffer(q)[0])
+            self.sims[q].halo_unpack(32 + p, *self.sims[p].halo_buffer(q))
         for s in self.sims:
             s.sync()
 
@@ -395,8 +395,8 @@ class Simulation:
     def halo_pack(self, slot):
         self._check(self.lib.rkdg_halo_pack(self.ctx, slot))
 
-    def halo_unpack(self, slot, device_buffer):
-        self._check(self.lib.rkdg_halo_unpack(self.ctx, slot, C.c_void_p(device_buffer)))
+    def halo_unpack(self, slot, device_buffer, nbytes):
+        self._check(self.lib.rkdg_halo_unpack(self.ctx, slot, C.c_void_p(device_buffer), nbytes))
 
     def state(self):
         """Modal state [3, ndof, ne] plus wet / limited [3, ne] / troubled flags."""

@@ -148,7 +148,7 @@ SIGNATURES = {
     "rkdg_halo_define": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int32]),
     "rkdg_halo_buffer": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_void_p), C.POINTER(C.c_int64)]),
     "rkdg_halo_pack": (C.c_int, [C.c_void_p, C.c_int32]),
-    "rkdg_halo_unpack": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p]),
+    "rkdg_halo_unpack": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_int64]),
     "rkdg_mesh_read_meta": (C.c_int, [C.c_char_p, C.c_char_p, C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),
     "rkdg_create_batch": (C.c_int, [C.c_int32, C.POINTER(MeshDesc), C.POINTER(Params), C.c_int,
                                     C.POINTER(C.c_void_p)]),

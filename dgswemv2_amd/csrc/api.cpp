@@ -60,7 +60,7 @@ struct EngineBase {
     // halo exchange of a partitioned mesh
     virtual rkdg_status halo_define(int slot, const int32_t* idx, int n) = 0;
     virtual rkdg_status halo_buffer(int slot, void** buf, int64_t* bytes) = 0;
-    virtual rkdg_status halo_move(int slot, bool pack, const void* src) = 0;
+    virtual rkdg_status halo_move(int slot, bool pack, const void* src, int64_t src_bytes) = 0;
     int block = 128;            // work-items per block (measured best for the stage kernels on 1M elements)
     int persistent_grid = 0;    // > 0: persistent stage kernel on this many work-groups (ensembles)
     bool capturing = false;     // inside a graph capture: no host-side event records
@@ -157,11 +157,13 @@ struct Engine final : EngineBase {
         if (bytes) *bytes = (int64_t)(3 * M.ndof + 1) * halo[slot].n * (int64_t)sizeof(Real);
         return RKDG_OK;
     }
-    rkdg_status halo_move(int slot, bool pack, const void* src) override {
+    rkdg_status halo_move(int slot, bool pack, const void* src, int64_t src_bytes) override {
         if (slot < 0 || slot >= (int)halo.size() || !halo[slot].idx)
             return fail("halo slot not defined", hipErrorInvalidValue), RKDG_ERR_ARG;
-        if (!pack && !src) return fail("null halo buffer", hipErrorInvalidValue), RKDG_ERR_ARG;
         const Halo& h = halo[slot];
+        if (!pack && !src) return fail("null halo buffer", hipErrorInvalidValue), RKDG_ERR_ARG;
+        if (!pack && src_bytes < (int64_t)(3 * M.ndof + 1) * h.n * (int64_t)sizeof(Real))
+            return fail("halo buffer smaller than the slot's data", hipErrorInvalidValue), RKDG_ERR_ARG;
         Real* cur = q[M.stepper.stage];
         Real* buf = pack ? h.buf : const_cast<Real*>(static_cast<const Real*>(src));
         hipError_t e = rkdg::launch_halo<Real>(pack, M.ne, 3 * M.ndof, cur, wet[wcur], h.idx, h.n, buf, stream);
@@ -914,14 +916,14 @@ rkdg_status rkdg_halo_buffer(rkdg_ctx* ctx, int32_t slot, void** device_buffer, 
 rkdg_status rkdg_halo_pack(rkdg_ctx* ctx, int32_t slot) {
     if (!ctx) return RKDG_ERR_ARG;
     if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no halo exchange");
-    rkdg_status s = ctx->eng->halo_move(slot, true, nullptr);
+    rkdg_status s = ctx->eng->halo_move(slot, true, nullptr, 0);
     return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
 }
 
-rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buffer) {
+rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buffer, int64_t bytes) {
     if (!ctx) return RKDG_ERR_ARG;
     if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: no halo exchange");
-    rkdg_status s = ctx->eng->halo_move(slot, false, device_buffer);
+    rkdg_status s = ctx->eng->halo_move(slot, false, device_buffer, bytes);
     return s == RKDG_OK ? s : set_err(ctx, s, ctx->eng->err);
 }
 

@@ -353,12 +353,12 @@ rkdg_status rkdg_mesh_renumber(const rkdg_mesh_desc* mesh, rkdg_mesh** out, rkdg
  * holds [3 ndof + 1][n] values of the context's precision (double for the FP64 forms, float for
  * FP32): the modes of the current state and the wet flag (0/1).  pack: current state -> buffer;
  * unpack: a buffer of the same layout (another context's slot, same precision and order) ->
- * current state of the listed elements.  Both are stream-ordered on ctx's stream; order them
- * across contexts with rkdg_sync. */
+ * current state of the listed elements (`bytes`: the source buffer's size, at least the slot's).
+ * Both are stream-ordered on ctx's stream; order them across contexts with rkdg_sync. */
 rkdg_status rkdg_halo_define(rkdg_ctx* ctx, int32_t slot, const int32_t* local_elements, int32_t n);
 rkdg_status rkdg_halo_buffer(rkdg_ctx* ctx, int32_t slot, void** device_buffer, int64_t* bytes);
 rkdg_status rkdg_halo_pack(rkdg_ctx* ctx, int32_t slot);
-rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buffer);
+rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buffer, int64_t bytes);
 
 /* ---- output ----------------------------------------------------------------------------
  * The reference's writers, at a step boundary, single-mesh contexts.

@@ -145,6 +145,10 @@ def test_gpu_halo_errors():
         s.halo_define(1, [5])
     buf, nbytes = s.halo_buffer(1)
     assert buf and nbytes == 2 * (3 * 6 + 1) * 8
+    with pytest.raises(RkdgError, match="smaller than"):
+        s.halo_unpack(1, buf, nbytes - 8)
+    s.halo_pack(1)
+    s.halo_unpack(1, buf, nbytes)  # its own data back in place
     s.close()
 
 
