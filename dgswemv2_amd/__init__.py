@@ -294,6 +294,13 @@ class PartitionedSimulation:
         for _ in range(nsteps * self.params.nstages):
             self.stage()
 
+    def set_state(self, q, wet=None):
+        """Set the whole mesh's state [3, ndof, ne] (and wet flags): each part takes its own elements
+        and its ghosts."""
+        q = np.asarray(q, np.float64)
+        for s, gid in zip(self.sims, self.gids):
+            s.set_state(np.ascontiguousarray(q[:, :, gid]), None if wet is None else np.asarray(wet)[gid])
+
     def state(self):
         """The whole mesh's state (q, wet) gathered from the parts' own elements."""
         ndof = (self.params.p + 1) * (self.params.p + 2) // 2

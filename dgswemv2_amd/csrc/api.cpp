@@ -140,7 +140,6 @@ struct Engine final : EngineBase {
         if (slot < 0 || slot >= (int)halo.size() || n < 0 || (n > 0 && !idx))
             return fail("bad halo slot", hipErrorInvalidValue), RKDG_ERR_ARG;
         if (M.ens.n > 0) return fail("no halo exchange for ensembles", hipErrorInvalidValue), RKDG_ERR_ARG;
-        if (!M.levee_gp.empty()) return fail("no halo exchange with levees", hipErrorInvalidValue), RKDG_ERR_ARG;
         for (int i = 0; i < n; ++i)
             if (idx[i] < 0 || idx[i] >= M.ne) return fail("halo element out of range", hipErrorInvalidValue), RKDG_ERR_ARG;
         Halo& h = halo[slot];

@@ -8,8 +8,11 @@
 // elements as ordinary elements in global element order: an element of the part then sees exactly
 // the operands it sees in the whole mesh (neighbour modes, which side of an interface is the
 // lower ID, the order of its faces), so a partitioned run equals the whole-mesh run bit for bit
-// once the ghosts hold their owners' current state before each stage.  Faces of a ghost that
-// lead out of the submesh are typed land; ghosts' own results are overwritten by the exchange.
+// once the ghosts hold their owners' current state before each stage.  Levee faces are interfaces
+// between the elements either side of the barrier (rkdg_swe_dbc_distributed_levee.hpp carries them
+// across parts in the reference), so the element across a levee is a ghost like any other
+// neighbour.  Faces of a ghost that lead out of the submesh (levee faces included) are typed land;
+// ghosts' own results are overwritten by the exchange.
 #include <algorithm>
 #include <array>
 #include <map>
@@ -89,7 +92,6 @@ extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part
     *out = nullptr;
     try {
         const long ne = d->n_elements, nn = d->n_nodes, nc = d->n_constituents;
-        if (d->n_levee_pairs > 0) throw PartError{"partitions of meshes with levees are not supported"};
         std::unordered_map<uint64_t, std::array<long, 2>> owners;  // edge -> its elements (-1: none)
         owners.reserve(3 * (size_t)ne);
         for (long e = 0; e < ne; ++e)
@@ -98,6 +100,29 @@ extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part
                                          std::array<long, 2>{e, -1});
                 if (!it.second) it.first->second[1] = e;
             }
+        std::unordered_map<uint64_t, int> btype;
+        for (long i = 0; i < d->n_bound_edges; ++i)
+            btype[ekey(d->bound_edge_nodes[2 * i], d->bound_edge_nodes[2 * i + 1])] = d->bound_edge_type[i];
+        // A levee face's neighbour is the element across the barrier: the one edge whose nodes are
+        // the barrier partners of the face's nodes (the levee interface of the whole mesh).
+        std::unordered_map<long, long> partner;
+        for (long i = 0; i < d->n_levee_pairs; ++i) {
+            const long f = d->levee_nodes[2 * i], b = d->levee_nodes[2 * i + 1];
+            if (f < 0 || f >= nn || b < 0 || b >= nn) throw PartError{"levee node out of range"};
+            partner[f] = b, partner[b] = f;
+        }
+        auto across = [&](long e, int k) -> long {  // neighbour of face k of e, levees included
+            const long a = d->elt_nodes[3 * e + (k + 1) % 3], b = d->elt_nodes[3 * e + (k + 2) % 3];
+            const auto& o = owners[ekey(a, b)];
+            if (o[1] >= 0) return o[0] == e ? o[1] : o[0];
+            auto t = btype.find(ekey(a, b));
+            if (t == btype.end() || t->second != RKDG_BC_LEVEE) return -1;
+            auto pa = partner.find(a), pb = partner.find(b);
+            if (pa == partner.end() || pb == partner.end()) throw PartError{"levee edge without barrier node pairs"};
+            auto x = owners.find(ekey(pa->second, pb->second));
+            if (x == owners.end() || x->second[1] >= 0) throw PartError{"levee edge without a single partner element"};
+            return x->second[0];
+        };
         // local elements: the part's own and their neighbours, in global order
         std::vector<char> local(ne, 0);
         long nown = 0;
@@ -105,8 +130,7 @@ extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part
             if (part[e] != p) continue;
             ++nown, local[e] = 1;
             for (int k = 0; k < 3; ++k) {
-                const auto& o = owners[ekey(d->elt_nodes[3 * e + (k + 1) % 3], d->elt_nodes[3 * e + (k + 2) % 3])];
-                const long nb = o[0] == e ? o[1] : o[0];
+                const long nb = across(e, k);
                 if (nb >= 0) local[nb] = 1;
             }
         }
@@ -128,29 +152,33 @@ extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part
             }
         for (int32_t e : M->global_elt)
             for (int k = 0; k < 3; ++k) M->elt.push_back((int32_t)nloc[d->elt_nodes[3 * e + k]]);
-        // boundary edges: the global ones among the local elements' faces, and land on ghost faces
-        // that lead out of the submesh
-        std::unordered_map<uint64_t, int> btype;
-        for (long i = 0; i < d->n_bound_edges; ++i)
-            btype[ekey(d->bound_edge_nodes[2 * i], d->bound_edge_nodes[2 * i + 1])] = d->bound_edge_type[i];
+        // boundary edges: the global ones among the local elements' faces (levee faces whose
+        // partner is local stay levees), and land on ghost faces that lead out of the submesh
         for (int32_t e : M->global_elt)
             for (int k = 0; k < 3; ++k) {
                 const long a = d->elt_nodes[3 * e + (k + 1) % 3], b = d->elt_nodes[3 * e + (k + 2) % 3];
-                const auto& o = owners[ekey(a, b)];
-                const long nb = o[0] == e ? o[1] : o[0];
+                const bool interior = owners[ekey(a, b)][1] >= 0;
+                const long nb = across(e, k);
                 int t;
-                if (nb < 0) {
+                if (nb >= 0 && !local[nb]) {
+                    t = RKDG_BC_LAND;  // only ghosts have neighbours outside the submesh
+                } else if (!interior) {
                     auto it = btype.find(ekey(a, b));
                     if (it == btype.end()) throw PartError{"boundary edge without a code"};
                     t = it->second;
-                } else if (!local[nb]) {
-                    t = RKDG_BC_LAND;  // only ghosts have neighbours outside the submesh
                 } else {
                     continue;
                 }
                 M->bedge.push_back((int32_t)nloc[a]), M->bedge.push_back((int32_t)nloc[b]);
                 M->btype.push_back((uint8_t)t);
             }
+        // barrier node pairs whose nodes are both in the submesh (those of the levee faces kept)
+        for (long i = 0; i < d->n_levee_pairs; ++i) {
+            const long f = d->levee_nodes[2 * i], b = d->levee_nodes[2 * i + 1];
+            if (!nused[f] || !nused[b]) continue;
+            M->levee_nodes.push_back((int32_t)nloc[f]), M->levee_nodes.push_back((int32_t)nloc[b]);
+            for (int c = 0; c < 3; ++c) M->levee_param.push_back(d->levee_param[3 * i + c]);
+        }
         if (nc > 0) {
             M->freq.assign(d->frequency, d->frequency + nc);
             M->ff.assign(d->forcing_fact, d->forcing_fact + nc);

@@ -322,8 +322,9 @@ rkdg_status rkdg_mesh_read_meta(const char* meta_path, const char* bcis_path, rk
  * run bit for bit in every precision.  With the slope limiter the exchange also runs between the
  * stage kernel and the limiter (rkdg_stage_begin / rkdg_stage_finish); the state then still equals
  * the whole-mesh run's, while the `troubled` output flags of elements near a cut may differ (the
- * trouble check's dilation does not cross parts; RKDG never reads those flags).  Not for levees
- * across parts, nor ensembles. */
+ * trouble check's dilation does not cross parts; RKDG never reads those flags).  Levees may cross
+ * parts: the element across a levee face is a ghost like an interface neighbour (the reference's
+ * distributed levee, rkdg_swe_dbc_distributed_levee.hpp).  Not for ensembles. */
 
 /* One stage in two halves, for a partitioned run with the slope limiter: rkdg_stage_begin runs
  * the stage kernel (the state then is the new stage's, before limiting), rkdg_stage_finish the
@@ -334,8 +335,10 @@ rkdg_status rkdg_stage_finish(rkdg_ctx* ctx);
 /* Recursive coordinate bisection of the element barycentres into nparts parts of equal size
  * (+-1), along the longer extent at each level: part[e] in [0, nparts). */
 rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* mesh, int32_t nparts, int32_t* part);
-/* Submesh of part p: its elements and their neighbours in other parts (ghosts), in global
- * element order; nodes in global order; ghost faces leading out of the submesh are land.
+/* Submesh of part p: its elements and their neighbours in other parts (ghosts; the element across
+ * a levee face counts as a neighbour), in global element order; nodes in global order; ghost faces
+ * leading out of the submesh are land (levee faces included); the barrier node pairs whose nodes
+ * are both in the submesh are kept.
  * *global_element (may be NULL) receives the global index of each local element (owned by the
  * returned mesh; free with rkdg_mesh_free). */
 rkdg_status rkdg_submesh(const rkdg_mesh_desc* mesh, const int32_t* part, int32_t p, rkdg_mesh** out,

@@ -62,11 +62,14 @@ def test_partition_and_submeshes(nparts):
                     assert key not in sbt
 
 
-def _oracle_partitioned(mesh, prm, nparts, stages):
+def _oracle_partitioned(mesh, prm, nparts, stages, part=None, q0=None):
     """The oracle stepping each submesh, ghosts refreshed from their owners after every stage."""
-    part = partition_rcb(mesh, nparts)
+    part = partition_rcb(mesh, nparts) if part is None else part
     subs = [submesh(mesh, part, p) for p in range(nparts)]
     sims = [OracleSim(s, prm) for s, _ in subs]
+    if q0 is not None:
+        for (_, gid), s in zip(subs, sims):
+            s.set_state(np.ascontiguousarray(q0[:, :, gid]))
     for _ in range(stages):
         for s in sims:
             s.stage()
@@ -124,9 +127,7 @@ def test_gpu_partitioned_equals_whole(precision, name, kw, nparts):
 @pytest.mark.gpu
 def test_gpu_halo_errors():
     from test_levee import weir_mesh, weir_params
-    s = Simulation(weir_mesh(), weir_params(), RKDG_FP64_PLAIN)  # levees: no halo exchange
-    with pytest.raises(RkdgError, match="levees"):
-        s.halo_define(0, [0, 1])
+    s = Simulation(weir_mesh(), weir_params(), RKDG_FP64_PLAIN)
     with pytest.raises(RkdgError, match="without stage_begin"):
         s.stage_finish()
     s.stage_begin()
@@ -150,6 +151,108 @@ def test_gpu_halo_errors():
     s.halo_pack(1)
     s.halo_unpack(1, buf, nbytes)  # its own data back in place
     s.close()
+
+
+# ---- levees across parts (the reference's distributed levee, rkdg_swe_dbc_distributed_levee.hpp)
+def _weir_cut():
+    """The weir fixture cut into 4 parts along the barrier (y = 15000 / 16000) and across it
+    (x = 9000): every barrier edge joins elements of different parts."""
+    from test_levee import weir_mesh
+    mesh = weir_mesh()
+    c = mesh.node_xyz[mesh.elt_nodes, :2].mean(axis=1)
+    return mesh, ((c[:, 1] > 15500) + 2 * (c[:, 0] > 9000)).astype(np.int32)
+
+
+def _weir_state(mesh, p):
+    """Still water at ze = 1.1 below the barrier and 0.2 above: the weir discharges from the start."""
+    from test_levee import _mode0_scale
+    nd = (p + 1) * (p + 2) // 2
+    up = mesh.node_xyz[mesh.elt_nodes, 1].mean(axis=1) < 15000
+    q = np.zeros((3, nd, mesh.n_elements))
+    q[0, 0] = np.where(up, 1.1, 0.2) * _mode0_scale(mesh, p)
+    return q
+
+
+def _levee_partners(mesh):
+    """(element, face) -> element across the barrier, from the barrier node pairs."""
+    edges, _ = _adjacency(mesh)
+    partner = {}
+    for f, b in mesh.levee_nodes.tolist():
+        partner[f], partner[b] = b, f
+    out = {}
+    for e, nodes in enumerate(mesh.elt_nodes.tolist()):
+        for k in range(3):
+            a, b = nodes[(k + 1) % 3], nodes[(k + 2) % 3]
+            if len(edges[frozenset((a, b))]) == 1 and a in partner and b in partner:
+                pe = edges.get(frozenset((partner[a], partner[b])))
+                if pe is not None:
+                    out[(e, k)] = pe[0]
+    return out
+
+
+def test_levee_submeshes():
+    mesh, part = _weir_cut()
+    lp = _levee_partners(mesh)
+    assert len(lp) == 36 and all(part[e] != part[x] for (e, _), x in lp.items())
+    _, nb = _adjacency(mesh)
+    for p in range(4):
+        sub, gid = submesh(mesh, part, p)
+        own = set(np.flatnonzero(part == p).tolist())
+        across = {x for (e, _), x in lp.items() if e in own}
+        assert across and set(gid.tolist()) == own | set().union(*(nb[e] for e in own)) | across
+        # a levee face stays a levee exactly when the element across it is in the submesh
+        slp = _levee_partners(sub)
+        kept = {(int(gid[e]), k): int(gid[x]) for (e, k), x in slp.items()}
+        want = {(e, k): x for (e, k), x in lp.items() if e in set(gid.tolist()) and x in set(gid.tolist())}
+        assert kept == want and all((e, k) in kept for (e, k) in lp if e in own)
+        ltype = {frozenset(map(int, be)): int(t) for be, t in zip(sub.bound_edges, sub.bound_types)}
+        for (e, k), x in _levee_partners(mesh).items():
+            if e in set(gid.tolist()) and x not in set(gid.tolist()):  # a ghost's barrier face
+                le = int(np.flatnonzero(gid == e)[0])
+                nodes = sub.elt_nodes[le].tolist()
+                assert part[e] != p and ltype[frozenset((nodes[(k + 1) % 3], nodes[(k + 2) % 3]))] == BC_LAND
+
+
+@pytest.mark.parametrize("cut", ["barrier", "rcb3"])
+def test_oracle_partitioned_levees_equal_whole(cut):
+    from test_levee import weir_params
+    mesh, part = _weir_cut()
+    if cut == "rcb3":
+        part = partition_rcb(mesh, 3)
+    prm = weir_params(p=1, dt=1.0, slope_limiting=False)
+    q0 = _weir_state(mesh, 1)
+    whole = OracleSim(mesh, prm)
+    whole.set_state(q0)
+    for _ in range(12):
+        whole.stage()
+    ref = whole.state()
+    whole.close()
+    assert np.abs(ref["q"][1:]).max() > 1e-3  # water moved over the barrier
+    q, _ = _oracle_partitioned(mesh, prm, int(part.max()) + 1, 12, part=part, q0=q0)
+    assert np.array_equal(q, ref["q"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", [RKDG_FP64_PLAIN, RKDG_FP64_FAST, RKDG_FP32_BULK])
+@pytest.mark.parametrize("cut,p,limiter", [("barrier", 1, False), ("barrier", 1, True), ("rcb3", 2, True)])
+def test_gpu_partitioned_levees_equal_whole(precision, cut, p, limiter):
+    from test_levee import weir_params
+    mesh, part = _weir_cut()
+    if cut == "rcb3":
+        part = partition_rcb(mesh, 3)
+    prm = weir_params(p=p, dt=1.0, slope_limiting=limiter)
+    q0 = _weir_state(mesh, p)
+    a = Simulation(mesh, prm, precision)
+    b = PartitionedSimulation(mesh, prm, precision, nparts=int(part.max()) + 1, part=part)
+    a.set_state(q0)
+    b.set_state(q0)
+    for k in range(6 * prm.nstages):
+        a.stage()
+        b.stage()
+        sa, sb = a.state(), b.state()
+        assert np.array_equal(sa["q"], sb["q"]) and np.array_equal(sa["wet"], sb["wet"]), k
+    assert np.abs(sa["q"][1:]).max() > 1e-4
+    a.close(), b.close()
 
 
 # ---- locality renumbering (rkdg_mesh_renumber)
