@@ -35,6 +35,17 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
+def rel_l2(a, b):
+    """Relative L2 error of a against b: per variable (ze, qx, qy), and of the momentum vector
+    (qx, qy) together; plus ||qy|| / ||qx|| of b, which says how far qy's own norm can be trusted as
+    a scale (a flow mostly along x leaves qy small, and its relative error large)."""
+    import numpy as np
+    per = [float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300)) for k in range(3)]
+    mom = float(np.linalg.norm(a[1:] - b[1:]) / max(np.linalg.norm(b[1:]), 1e-300))
+    ratio = float(np.linalg.norm(b[2]) / max(np.linalg.norm(b[1]), 1e-300))
+    return per, mom, ratio
+
+
 def dof_stages(ne, ndof, stages):
     return ne * ndof * 3 * stages
 
@@ -131,7 +142,7 @@ def learned_bench(steps, scale, warm=40, judge_steps=10):
     from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_PLAIN, Simulation, indicator as IND, workloads as W
 
     def rel(a, b):
-        return [float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300)) for k in range(3)]
+        return rel_l2(a, b)[0]
 
     log("building the dam-break mesh, scale", scale)
     mesh, prm = W.dambreak(scale=scale)
@@ -174,6 +185,10 @@ def learned_bench(steps, scale, warm=40, judge_steps=10):
     res["fp32_plain"]["rel_l2_vs_fp64_plain"] = rel(qs["fp32_plain"], qs["fp64_plain"])
     res["fp32_learned"]["rel_l2_vs_fp64_plain"] = rel(qs["fp32_learned"], qs["fp64_plain"])
     res["fp32_learned"]["rel_l2_vs_fp32_plain"] = rel(qs["fp32_learned"], qs["fp32_plain"])
+    for name in ("fp32_plain", "fp32_learned"):
+        _, mom, ratio = rel_l2(qs[name], qs["fp64_plain"])
+        res[name]["rel_l2_momentum_vs_fp64_plain"] = mom
+    res["qy_over_qx_norm"] = ratio
     b.set_indicator(w, threshold=thr, judge=True)
     b.step(judge_steps)
     js = b.indicator_stats()
@@ -311,11 +326,12 @@ def main():
         log("single form timed; no JSON line")
         return
     # errors against the FP64 plain form after the same steps from the same start state
-    def rel(a, b):
-        return [float(np.linalg.norm(a[k] - b[k]) / max(np.linalg.norm(b[k]), 1e-300)) for k in range(3)]
     gates = {"fp32_bulk": 1e-6, "fp64_fast": 1e-12}
     for name in ("fp64_fast", "fp32_bulk"):
-        forms[name]["rel_l2_vs_fp64_plain"] = rel(qs[name], qs["fp64_plain"])
+        per, mom, ratio = rel_l2(qs[name], qs["fp64_plain"])
+        forms[name]["rel_l2_vs_fp64_plain"] = per
+        forms[name]["rel_l2_momentum_vs_fp64_plain"] = mom
+        forms[name]["qy_over_qx_norm"] = ratio
         forms[name]["gate"] = gates[name]
         forms[name]["within_gate"] = max(forms[name]["rel_l2_vs_fp64_plain"]) <= gates[name]
     head = "fp64_plain"

@@ -203,6 +203,8 @@ def submesh(mesh, part, p):
     each local element."""
     lib = N.load_library()
     part = np.ascontiguousarray(part, dtype=np.int32)
+    if part.shape != (mesh.n_elements,):
+        raise ValueError(f"part has shape {part.shape}, expected ({mesh.n_elements},)")
     handle, desc, gid = C.c_void_p(), N.MeshDesc(), C.POINTER(C.c_int32)()
     st = lib.rkdg_submesh(C.byref(mesh.desc), N.iptr(part), p, C.byref(handle), C.byref(desc), C.byref(gid))
     if st != N.RKDG_OK:

@@ -62,6 +62,26 @@ def test_partition_and_submeshes(nparts):
                     assert key not in sbt
 
 
+def test_partition_errors():
+    from dgswemv2_amd import Mesh
+    from test_levee import weir_mesh
+    mesh, _ = W.tidal(scale=0.03)
+    for bad in (0, mesh.n_elements + 1):
+        with pytest.raises(RkdgError, match="bad arguments"):
+            partition_rcb(mesh, bad)
+    part = partition_rcb(mesh, 3)
+    with pytest.raises(RkdgError, match="part 3 has no elements"):
+        submesh(mesh, part, 3)
+    with pytest.raises(ValueError, match="shape"):
+        submesh(mesh, part[:-1], 0)
+    # a barrier edge whose nodes have no barrier partners
+    m = weir_mesh()
+    cut = Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, m.bound_types, levee_nodes=m.levee_nodes[:-3],
+               levee_param=m.levee_param[:-3])
+    with pytest.raises(RkdgError, match="levee edge without barrier node pairs"):
+        submesh(cut, np.zeros(m.n_elements, np.int32), 0)
+
+
 def _oracle_partitioned(mesh, prm, nparts, stages, part=None, q0=None):
     """The oracle stepping each submesh, ghosts refreshed from their owners after every stage."""
     part = partition_rcb(mesh, nparts) if part is None else part
