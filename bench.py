@@ -86,6 +86,24 @@ def cpu_baseline(budget_s=12.0):
                       f"{el:.1f} s, FP64 oracle with OpenMP over {threads} threads (coloured interface loop)"}
 
 
+def gate_probe(steps, scale):
+    """Only the learned gate's inference kernel: the dam break in FP32, 20 steps with the plain
+    limiter, then `steps` steps with a gate of seeded untrained weights (a counter run on the gate
+    kernel needs no training; its work per element does not depend on the weights)."""
+    from dgswemv2_amd import RKDG_FP32_BULK, Simulation, indicator as IND, workloads as W
+    mesh, prm = W.dambreak(scale=scale)
+    sim = Simulation(mesh, prm, RKDG_FP32_BULK)
+    sim.step(20)
+    sim.set_indicator(IND.init_weights(11), threshold=0.0)
+    sim.sync()
+    sim.reset_timers()
+    sim.step(steps)
+    sim.sync()
+    rep = sim.report()
+    sim.close()
+    log(f"gate probe: {mesh.n_elements} triangles, {steps} gated steps", json.dumps(rep))
+
+
 def copy_probe(mib):
     """Five plain device-to-device copies of a known size: the HBM counters of this kernel
     (FETCH_SIZE, WRITE_SIZE per dispatch) calibrate the counters read on the stage kernel."""
@@ -267,6 +285,8 @@ def main():
                     help="time one form only (short profiled runs); the JSON line needs all")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (counter calibration) and exit")
+    ap.add_argument("--gate-probe", type=int, default=0, metavar="STEPS",
+                    help="only run the learned gate's inference kernel for STEPS steps on the dam break and exit")
     ap.add_argument("--order", choices=("file", "shuffled", "hilbert"), default="file",
                     help="tidal mesh element order: as generated, random, or random then Hilbert-renumbered")
     ap.add_argument("--halo-parts", type=int, default=0, metavar="N",
@@ -274,6 +294,9 @@ def main():
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.gate_probe:
+        gate_probe(args.gate_probe, args.learned_scale)
         return
     if args.halo_parts:
         r = halo_bench(args.halo_parts, args.steps, args.warmup, args.scale)
