@@ -1,0 +1,163 @@
+"""Closed-form checks of the oracle's bottom friction and tide / flow boundary forcing.
+
+No reference test or stored trajectory covers these branches (SWE::get_source friction,
+swe_source.hpp:29-51, with the element Manning factor of swe_pre_init_data.hpp:166-197;
+BC::Tide, rkdg_swe_bc_tide.hpp:70-96; BC::Flow, rkdg_swe_bc_flow.hpp:72-101; both through
+LLF_flux, rkdg_swe_LLF_flux.hpp:7-60).  One forward-Euler stage fixes them in closed form:
+
+- uniform flow over a flat bottom: the volume and face integrals of a constant flux cancel exactly
+  in every element without a boundary face (the rules are exact for these degrees), so the stage
+  changes only the momentum, by the friction law: q1 = q0 (1 - dt Cf |u| / h), Cf = Cf_in (Chezy)
+  or max(g nbar^2 / h^(1/3), Cf_in) (Manning, nbar the mean of the element's nodal n);
+- still water with tide or flow boundaries: interior faces cancel and land faces carry no mass,
+  so the change of the total volume is minus dt times the boundary mass flux, integrated with the
+  (p+1)-point Gauss-Legendre rule over each forced edge at the stage time t = dt:
+  1/2 lambda (ze_in - ze_ex) with lambda = max(sqrt(g h_in), sqrt(g h_ex)) for the tide, 1/2 qn
+  for the flow, where ze_ex / qn = ramp sum_c ff_c A_c cos(w_c t + (eq_c - phi_c) pi/180) with A
+  and phi interpolated linearly along the edge.
+
+Each check also evaluates a plausible mistake (mean of n^2 instead of the squared mean, h^(4/3),
+the phase sign, a missing ramp) and requires it to disagree, so the checks can tell them apart.
+"""
+import numpy as np
+import pytest
+
+from dgswemv2_amd import (BC_FLOW, BC_LAND, BC_TIDE, FRICTION_CHEZY, FRICTION_MANNING, Mesh, make_params,
+                          quad_mesh)
+from oracle.oracle_py import OracleSim
+from test_levee import _areas, _mode0_scale
+
+G = 9.81
+PI = 3.14159265359  # the reference's PI (general_definitions.hpp)
+
+
+def _interior(mesh):
+    """Elements none of whose faces is a boundary edge."""
+    bnd = {frozenset(map(int, be)) for be in mesh.bound_edges}
+    out = []
+    for e, nodes in enumerate(mesh.elt_nodes.tolist()):
+        if all(frozenset((nodes[(k + 1) % 3], nodes[(k + 2) % 3])) not in bnd for k in range(3)):
+            out.append(e)
+    return np.array(out)
+
+
+def _friction_stage(h, friction, cf, n_range, p=1, dt=5.0, qx=0.6, qy=-0.3):
+    m = quad_mesh(0.0, 0.0, 8000.0, 6000.0, 8, 6, bathy="constant", b0=h, jitter=0.2, seed=4)
+    nn = None
+    if n_range is not None:
+        nn = np.random.default_rng(7).uniform(*n_range, size=len(m.node_xyz))
+        m = Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, m.bound_types, node_manning_n=nn)
+    prm = make_params(p=p, nstages=1, order=1, dt=dt, g=G, ic_constant=True, ze0=0.0, qx0=qx, qy0=qy,
+                      friction=friction, friction_cf=cf)
+    o = OracleSim(m, prm)
+    s0 = o.state()["q"].copy()
+    o.stage()
+    s1 = o.state()["q"].copy()
+    o.close()
+    return m, nn, s0, s1
+
+
+def _expected_factor(m, nn, h, cf, dt, speed, manning, nbar_sq=None, hpow=1.0 / 3.0):
+    if not manning:
+        Cf = np.full(m.n_elements, cf)
+    else:
+        n3 = nn[m.elt_nodes]
+        nsq = n3.mean(axis=1) ** 2 if nbar_sq is None else nbar_sq(n3)
+        Cf = np.maximum(G * nsq / h ** hpow, cf)
+    return 1.0 - dt * Cf * speed / h
+
+
+@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("case", ["chezy", "manning", "manning_floor"])
+def test_friction_uniform_flow(case, p):
+    h, dt, qx, qy = (10.0, 5.0, 0.6, -0.3) if case != "manning_floor" else (50.0, 50.0, 2.0, 1.0)
+    friction = FRICTION_CHEZY if case == "chezy" else FRICTION_MANNING
+    cf = {"chezy": 3e-3, "manning": 1e-4, "manning_floor": 1e-3}[case]
+    n_range = None if case == "chezy" else ((0.02, 0.04) if case == "manning" else (0.005, 0.03))
+    m, nn, s0, s1 = _friction_stage(h, friction, cf, n_range, p=p, dt=dt, qx=qx, qy=qy)
+    inner = _interior(m)
+    assert len(inner) > m.n_elements // 2
+    speed = np.hypot(qx, qy) / h
+    want = _expected_factor(m, nn, h, cf, dt, speed, friction == FRICTION_MANNING)[inner]
+    for k in (1, 2):
+        got = s1[k, 0, inner] / s0[k, 0, inner]
+        assert np.max(np.abs(got - want)) <= 1e-13, (k, np.max(np.abs(got - want)))
+        # the higher modes stay zero and the elevation does not move
+        assert np.max(np.abs(s1[k, 1:, inner])) <= 1e-11 * np.max(np.abs(s1[k, 0, inner]))
+    assert np.max(np.abs(s1[0, :, inner] - s0[0, :, inner])) <= 1e-14
+    # friction acts: the factor differs from 1 by far more than the tolerance
+    assert np.min(1.0 - want) > 1e-6
+    if case == "manning_floor":  # both branches of the floor occur
+        raw = G * nn[m.elt_nodes].mean(axis=1) ** 2 / h ** (1.0 / 3.0)
+        assert np.any(raw[inner] < cf) and np.any(raw[inner] > cf)
+    if friction == FRICTION_MANNING:  # plausible mistakes disagree
+        got = s1[1, 0, inner] / s0[1, 0, inner]
+        alt1 = _expected_factor(m, nn, h, cf, dt, speed, True, nbar_sq=lambda n3: (n3 ** 2).mean(axis=1))[inner]
+        alt2 = _expected_factor(m, nn, h, cf, dt, speed, True, hpow=4.0 / 3.0)[inner]
+        assert np.max(np.abs(got - alt1)) > 1e-11 and np.max(np.abs(got - alt2)) > 1e-9
+    # elements with a land face do feel the walls (the interior selection matters)
+    edge = np.setdiff1d(np.arange(m.n_elements), inner)
+    assert np.max(np.abs(s1[0, :, edge] - s0[0, :, edge])) > 1e-8
+
+
+def _forced_mesh(code, p, amp_range, seed):
+    base = quad_mesh(0.0, 0.0, 6000.0, 4000.0, 6, 4, bc=(BC_LAND, code, BC_LAND, BC_LAND), bathy="constant",
+                     b0=5.0, jitter=0.2, seed=seed)
+    rng = np.random.default_rng(seed)
+    nv = len(base.node_xyz)
+    amp, ph = rng.uniform(*amp_range, size=nv), rng.uniform(0.0, 60.0, size=nv)
+    freq, ff, eq = 2e-3, 0.9, 25.0
+    m = Mesh(base.node_xyz, base.elt_nodes, base.bound_edges, base.bound_types, frequency=[freq],
+             forcing_fact=[ff], equilib_arg=[eq], node_amplitude=amp, node_phase=ph)
+    return m, amp, ph, freq, ff, eq
+
+
+def _boundary_volume_rate(m, code, amp, ph, freq, ff, eq, t, ramp, p, ze0, H, phase_sign=1.0):
+    """Outward boundary mass flux (m^3/s) from still water at ze0 over every edge of `code`."""
+    z, w = np.polynomial.legendre.leggauss(p + 1)
+    total = 0.0
+    for (a, b), c in zip(m.bound_edges.tolist(), m.bound_types.tolist()):
+        if c != code:
+            continue
+        L = np.hypot(*(m.node_xyz[b, :2] - m.node_xyz[a, :2]))
+        A = amp[a] * (1 - z) / 2 + amp[b] * (1 + z) / 2
+        P = ph[a] * (1 - z) / 2 + ph[b] * (1 + z) / 2
+        forcing = ramp * ff * A * np.cos(freq * t + phase_sign * (eq - P) * PI / 180)
+        if code == BC_TIDE:
+            lam = np.maximum(np.sqrt(G * (ze0 + H)), np.sqrt(G * (forcing + H)))
+            F = 0.5 * lam * (ze0 - forcing)
+        else:
+            F = 0.5 * forcing
+        total += L / 2 * np.sum(w * F)
+    return total
+
+
+@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("code", [BC_TIDE, BC_FLOW])
+def test_boundary_forcing_mass_flux(code, p):
+    H, ze0, dt = 5.0, 0.3, 100.0
+    ramp_days = 0.01 if code == BC_TIDE else 0.0
+    m, amp, ph, freq, ff, eq = _forced_mesh(code, p, (0.2, 0.8) if code == BC_TIDE else (0.1, 0.5), seed=5 + p)
+    prm = make_params(p=p, nstages=1, order=1, dt=dt, g=G, ic_constant=True, ze0=ze0, ramp_duration=ramp_days)
+    o = OracleSim(m, prm)
+    o.step(1)  # the stage below runs at t = dt
+    c = _mode0_scale(m, p)
+    still = np.zeros_like(o.state()["q"])
+    still[0, 0] = ze0 * c
+    o.set_state(still)
+    area = _areas(m)
+    v0 = np.sum(o.state()["q"][0, 0] / c * area)
+    o.stage()
+    v1 = np.sum(o.state()["q"][0, 0] / c * area)
+    o.close()
+    t = dt
+    ramp = np.tanh(2 * (t / 86400) / ramp_days) if ramp_days else 1.0
+    want = -dt * _boundary_volume_rate(m, code, amp, ph, freq, ff, eq, t, ramp, p, ze0, H)
+    assert abs(want) > 1.0
+    assert abs((v1 - v0) - want) <= 1e-9 * abs(want), (v1 - v0, want)
+    # plausible mistakes disagree: the phase sign, and (tide) the ramp left out
+    alt = -dt * _boundary_volume_rate(m, code, amp, ph, freq, ff, eq, t, ramp, p, ze0, H, phase_sign=-1.0)
+    assert abs(alt - want) > 1e-4 * abs(want)
+    if ramp_days:
+        alt = -dt * _boundary_volume_rate(m, code, amp, ph, freq, ff, eq, t, 1.0, p, ze0, H)
+        assert abs(alt - want) > 1e-2 * abs(want)
