@@ -14,11 +14,11 @@
 // (test_llf_flux.cpp), element integration golden vectors (test_element_triangle.hpp),
 // the five wetting-drying scenarios (test_wetting_drying.cpp), the RK ODE test
 // (test_rk_stepper.cpp), plus invariants (lake at rest, mass conservation, manufactured-
-// solution convergence, limiter invariants), and closed forms of one forward-Euler stage for
-// bottom friction and tide / flow forcing (tests/test_oracle_forcing.py).  Parity unpinned: the
-// trouble check (the `troubled` flag, never read by RKDG) and whole-run trajectories, which no
-// reference test or stored output covers and which cannot be compared against the reference
-// binary (it does not build here); see the test headers.
+// solution convergence, limiter invariants), closed forms of one forward-Euler stage for
+// bottom friction and tide / flow forcing (tests/test_oracle_forcing.py), and of the trouble
+// check on a step (tests/test_oracle_trouble.py).  Parity unpinned: whole-run trajectories,
+// which no reference test or stored output covers and which cannot be compared against the
+// reference binary (it does not build here); see the test headers.
 //
 // Third-party arithmetic restated: Eigen 3.3.4 (not vendored in the reference) 2x2
 // determinant a00*a11 - a10*a01 and inverse (1/det then cofactor*invdet), and the
