@@ -161,3 +161,34 @@ def test_boundary_forcing_mass_flux(code, p):
     if ramp_days:
         alt = -dt * _boundary_volume_rate(m, code, amp, ph, freq, ff, eq, t, 1.0, p, ze0, H)
         assert abs(alt - want) > 1e-2 * abs(want)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [("plain", 1e-13), ("fast", 1e-13), ("fp32", 2e-6)])
+@pytest.mark.parametrize("case", ["chezy", "manning", "manning_floor"])
+def test_gpu_friction_uniform_flow(case, precision, tol):
+    """The engine's friction on the same uniform flows (p = 2), each precision to its rounding."""
+    from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, Simulation
+    prec = {"plain": RKDG_FP64_PLAIN, "fast": RKDG_FP64_FAST, "fp32": RKDG_FP32_BULK}[precision]
+    h, dt, qx, qy = (10.0, 5.0, 0.6, -0.3) if case != "manning_floor" else (50.0, 50.0, 2.0, 1.0)
+    friction = FRICTION_CHEZY if case == "chezy" else FRICTION_MANNING
+    cf = {"chezy": 3e-3, "manning": 1e-4, "manning_floor": 1e-3}[case]
+    n_range = None if case == "chezy" else ((0.02, 0.04) if case == "manning" else (0.005, 0.03))
+    m = quad_mesh(0.0, 0.0, 8000.0, 6000.0, 8, 6, bathy="constant", b0=h, jitter=0.2, seed=4)
+    nn = None
+    if n_range is not None:
+        nn = np.random.default_rng(7).uniform(*n_range, size=len(m.node_xyz))
+        m = Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, m.bound_types, node_manning_n=nn)
+    prm = make_params(p=2, nstages=1, order=1, dt=dt, g=G, ic_constant=True, ze0=0.0, qx0=qx, qy0=qy,
+                      friction=friction, friction_cf=cf)
+    sim = Simulation(m, prm, prec)
+    s0 = sim.state()["q"].copy()
+    sim.stage()
+    s1 = sim.state()["q"].copy()
+    sim.close()
+    inner = _interior(m)
+    want = _expected_factor(m, nn, h, cf, dt, np.hypot(qx, qy) / h, friction == FRICTION_MANNING)[inner]
+    assert np.min(1.0 - want) > 10 * tol
+    for k in (1, 2):
+        got = s1[k, 0, inner] / s0[k, 0, inner]
+        assert np.max(np.abs(got - want)) <= tol, (k, np.max(np.abs(got - want)))

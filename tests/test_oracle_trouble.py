@@ -96,3 +96,24 @@ def test_trouble_check_small_step_and_flat(p):
     for zl, zr in ((0.3, 0.301), (0.3, 0.3)):
         troubled, _ = _troubled_after_step(mesh, p, zl, zr, 0.6)
         assert not troubled.any(), (zl, zr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["plain", "fast", "fp32"])
+@pytest.mark.parametrize("zl,zr,side", [(0.1, 0.6, "right"), (-0.2, -0.6, "left"), (0.3, 0.301, None)])
+def test_gpu_trouble_check_step(precision, zl, zr, side):
+    """The engine's trouble check on the same steps (p = 2): the same closed-form sets."""
+    from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, Simulation
+    prec = {"plain": RKDG_FP64_PLAIN, "fast": RKDG_FP64_FAST, "fp32": RKDG_FP32_BULK}[precision]
+    mesh = quad_mesh(0.0, 0.0, 1.2, 0.8, 12, 8, bathy="constant", b0=1.0)
+    prm = make_params(p=2, nstages=1, order=1, dt=1e-7, g=9.81, slope_limiting=True, sl_M=1e-8, sl_nu=1.5)
+    cx = mesh.node_xyz[mesh.elt_nodes, 0].mean(axis=1)
+    sim = Simulation(mesh, prm, prec)
+    q = np.zeros_like(sim.state()["q"])
+    q[0, 0] = np.where(cx < 0.6, zl, zr) * _mode0_scale(mesh, 2)
+    sim.set_state(q)
+    sim.stage()
+    got = set(np.flatnonzero(sim.state()["troubled"]).tolist())
+    sim.close()
+    want = set() if side is None else _ball(_neighbours(mesh), _step_side(mesh, cx, 0.6, side == "right"), 5)
+    assert got == want
