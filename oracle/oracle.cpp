@@ -16,9 +16,10 @@
 // (test_rk_stepper.cpp), plus invariants (lake at rest, mass conservation, manufactured-
 // solution convergence, limiter invariants), closed forms of one forward-Euler stage for
 // bottom friction and tide / flow forcing (tests/test_oracle_forcing.py), and of the trouble
-// check on a step (tests/test_oracle_trouble.py).  Parity unpinned: whole-run trajectories,
-// which no reference test or stored output covers and which cannot be compared against the
-// reference binary (it does not build here); see the test headers.
+// check on a step (tests/test_oracle_trouble.py).  Parity unpinned: whole-run trajectories of
+// the bowl, tidal and dam-break workloads beyond those invariants, which no reference test or
+// stored output covers and which cannot be compared against the reference binary (it does not
+// build here); see the test headers.
 //
 // Third-party arithmetic restated: Eigen 3.3.4 (not vendored in the reference) 2x2
 // determinant a00*a11 - a10*a01 and inverse (1/det then cofactor*invdet), and the

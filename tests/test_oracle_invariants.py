@@ -4,7 +4,8 @@ exists to compare against -- the reference's CPU path does not build here).
   lake at rest: ze = c, q = 0 over piecewise-linear bathymetry stays at rest (quadrature exact)
   mass: with land boundaries total mass changes only by roundoff
   manufactured solution: second-order convergence when dt ~ h^2 (the reference's drk = 0
-    stage time makes forcing first order in dt, so dt must shrink like h^2)
+    stage time makes forcing first order in dt, so dt must shrink like h^2); p = 2 third order
+    with dt ~ h^3, and first order with dt ~ h (the drk = 0 quirk seen in whole runs)
   limiter: cell means are preserved; a piecewise-constant field is never limited
   OpenMP (coloured interfaces) agrees with serial to roundoff
 """
@@ -59,6 +60,23 @@ def test_manufactured_convergence():
         errs.append(o.l2_residual())
     r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
     assert 3.3 < r1 < 4.7 and 3.5 < r2 < 4.5, errs
+
+
+def test_manufactured_convergence_p2():
+    """p = 2, SSP-RK3: third order once dt shrinks like h^3 (error ratio 8 per halving of h);
+    with dt ~ h the error halves with h instead, because the stage time t + dt drk[s] has
+    drk = 0 (explicit_ssp_rk_stepper.cpp:272-276), which leaves the forcing first order in dt.  A
+    stepper with the textbook stage times would give about 8 in both studies."""
+    def err(scale, dt, t_end):
+        mesh, prm = W.manufactured(scale=scale, p=2, nstages=3, order=3, dt=dt)
+        o = OracleSim(mesh, prm, threads=4)
+        o.step(int(round(t_end / dt)))
+        e = o.l2_residual()
+        o.close()
+        return e
+    r3 = err(0.25, 5e-4, 0.02) / err(0.5, 6.25e-5, 0.02)
+    r1 = err(0.25, 2e-3, 0.1) / err(0.5, 1e-3, 0.1)
+    assert 7.0 < r3 < 9.0 and 1.7 < r1 < 2.5, (r3, r1)
 
 
 def test_manufactured_convergence_catches_planted_error():
