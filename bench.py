@@ -141,12 +141,11 @@ def ensemble_bench(steps, warmup, scale):
     log("ensemble autotune", best, "ms/step", tuned_ms)
     b.step(warmup)
     b.sync()
-    b.reset_timers()
     t0 = time.perf_counter()
     b.step(steps)
     b.sync()
     wall = time.perf_counter() - t0
-    rep = b.report()
+    rep = kernel_window(b, steps)
     ns = params[0].nstages
     ms_step = rep["total_ms"] / steps
     achieved = rep["stage_bytes_alg"] * ns / (ms_step * 1e-3) / 1e9
