@@ -312,8 +312,8 @@ def main():
         log("halo", json.dumps(r))
         return
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the engine's multi-GPU sharding is "
-              "described in DESIGN.md and not built", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}): the partitioned path runs its parts on "
+              "one GPU (--halo-parts N); moving halo buffers between GPUs is not built", file=sys.stderr)
         sys.exit(2)
 
     from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, workloads as W
