@@ -156,7 +156,7 @@ def ensemble_bench(steps, warmup, scale):
             "dtype": "f32"}
 
 
-def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5):
+def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5, drift_steps=100):
     """Learned troubled-cell indicator on the 500k-triangle dam break (p = 1, SSP-RK2, wetting-drying,
     Cockburn-Shu limiter).  FP32 bulk with the plain limiter runs `warm` steps with capture on
     (features and labels of every limiter stage), the MLP is trained on the GPU and refined by one
@@ -165,7 +165,8 @@ def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5):
     learned gate; L2 errors are taken against FP64 plain at the end of those steps.  Per-kernel
     durations come from `kernel_steps` further steps with events on.  A further `judge_steps`
     window with the judge on (the plain limiter's decision on the learned run's states) gives the
-    miss rate."""
+    miss rate; then `drift_steps` more steps with the gate fixed and with online retraining, each
+    judged the same way."""
     import numpy as np
     from dgswemv2_amd import RKDG_FP32_BULK, RKDG_FP64_PLAIN, Simulation, indicator as IND, workloads as W
 
@@ -189,7 +190,8 @@ def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5):
     w, thr, info, X, y = IND.refine(lambda: Simulation(mesh, prm, RKDG_FP32_BULK), warm, w, thr, X, y, epochs=3,
                                     batch=1024, pos_weight=8.0, recall=0.9999, margin=16.0)
     refine_s = time.perf_counter() - t0
-    del X, y
+    keep = 4_000_000  # the newest samples seed the online retraining buffer below
+    X, y = np.ascontiguousarray(X[:, -keep:]), np.ascontiguousarray(y[-keep:])
     log("refined in", round(refine_s, 2), "s:", json.dumps(info))
     b = Simulation(mesh, prm, RKDG_FP32_BULK)
     b.step(warm)
@@ -219,8 +221,39 @@ def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5):
     b.set_indicator(w, threshold=thr, judge=True)
     b.step(judge_steps)
     js = b.indicator_stats()
-    for sim in (a, b, r):
+    for sim in (a, r):
         sim.close()
+    # drift: `drift_steps` more steps from here with the gate fixed, and with online retraining
+    # (IND.run_online: every 20 steps one captured step, labelled by the judge, and 2 epochs from
+    # the current weights), each judged over the `judge_steps` after them
+    st = b.state()
+    b.close()
+    drift = {"steps": drift_steps}
+    for mode in ("fixed", "online"):
+        s = Simulation(mesh, prm, RKDG_FP32_BULK)
+        s.set_state(st["q"], st["wet"])
+        t0 = time.perf_counter()
+        if mode == "fixed":
+            s.set_indicator(w, threshold=thr)
+            s.step(drift_steps)
+            w2, thr2, rounds = w, thr, []
+        else:
+            w2, thr2, X, y, rounds = IND.run_online(s, w, thr, drift_steps, X=X, y=y, every=20, window=1, epochs=2,
+                                                    keep=keep, batch=1024, pos_weight=8.0, recall=0.9999,
+                                                    margin=16.0)
+        s.sync()
+        wall = time.perf_counter() - t0
+        s.set_indicator(w2, threshold=thr2, judge=True)
+        s.step(judge_steps)
+        jd = s.indicator_stats()
+        s.close()
+        drift[mode] = {"wall_s": wall, "retrain_rounds": len(rounds), "gate_open": jd["flagged"],
+                       "candidates": jd["candidates"], "significant_write_backs": jd["significant"],
+                       "missed_significant": jd["missed_significant"],
+                       "missed_share": jd["missed_significant"] / max(1, jd["significant"]),
+                       "max_missed_change": jd["max_missed_change"]}
+        log("drift", mode, json.dumps(drift[mode]))
+    del X, y
     return {"workload": f"dam break p=1 SSP-RK2, {ne} triangles (500x500 quads over 5 km, 10 m / 1 m step, dry "
                         "floodplain, wetting-drying, Cockburn-Shu limiter)", "triangles": ne, "warm_steps": warm,
             "timed_steps": steps, "dtype": "f32 (FP64 limiter arithmetic), bf16 MFMA gate", **res,
@@ -234,7 +267,8 @@ def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5):
                       "significant_write_backs": js["significant"],
                       "missed_significant": js["missed_significant"],
                       "missed_share": js["missed_significant"] / max(1, js["significant"]),
-                      "max_missed_change": js["max_missed_change"]}}
+                      "max_missed_change": js["max_missed_change"]},
+            "drift": drift}
 
 
 def halo_bench(nparts, steps, warmup, scale):

@@ -12,7 +12,8 @@ dgswemv2_amd/csrc/indicator.hpp.  The workflow here:
   4. Simulation.set_indicator(weights, threshold): the limiter runs only where the gate opens;
   5. optionally one aggregation round (refine): replay the training window under the learned gate
      with capture on (labels then come from the judge's dry run of the plain limit pass on the
-     states the gate itself produces), add those samples and retrain from the current weights.
+     states the gate itself produces), add those samples and retrain from the current weights;
+  6. optionally keep retraining during the run (run_online) as the flow leaves the training window.
 Only elements where the limiter may act (element and neighbours wet) are training samples;
 elsewhere the gate is closed by construction.
 """
@@ -121,3 +122,33 @@ def refine(make_sim, steps, w, thr, X, y, epochs=3, seed=2, **kw):
     info["refine_samples"] = int(yd.size)
     return w, thr, info, X, y
 
+
+
+def run_online(sim, w, thr, steps, X=None, y=None, every=20, window=1, epochs=2, keep=4_000_000, neg_keep=0.25,
+               seed=3, **kw):
+    """Steps `sim` `steps` steps under the gate (w, thr) and retrains on judged data as it goes:
+    after every `every` gated steps, `window` more steps run with capture on (their labels are the
+    judge's dry run of the plain limit pass on the states the gate produces), the samples join a
+    buffer that starts from (X, y) and keeps the newest `keep`, and the network is retrained from its
+    current weights for `epochs` epochs (train's keyword arguments in **kw).  Returns (weights,
+    threshold, X, y, rounds) with one info dict per retraining round."""
+    rounds = []
+    sim.set_indicator(w, threshold=thr)
+    done = 0
+    while done < steps:
+        n = min(every, steps - done)
+        sim.step(n)
+        done += n
+        if done >= steps:
+            break
+        n = min(window, steps - done)
+        Xd, yd, _ = collect(sim, n, every=1, neg_keep=neg_keep, seed=seed + done)
+        done += n
+        X = Xd if X is None else np.concatenate([X, Xd], axis=1)[:, -keep:]
+        y = yd if y is None else np.concatenate([y, yd])[-keep:]
+        w, thr, info = train(X, y, epochs=epochs, init=w, seed=seed, **kw)
+        info["step"] = done
+        info["new_samples"] = int(yd.size)
+        rounds.append(info)
+        sim.set_indicator(w, threshold=thr)
+    return w, thr, X, y, rounds

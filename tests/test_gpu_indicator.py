@@ -8,7 +8,9 @@
   plain one where it runs);
 - the trainer's forward pass and gradient vs the oracle; training is deterministic and learns;
 - what the trained network gains on the dam break: limiter time, misses, L2 against the plain
-  limiter.
+  limiter;
+- retraining during the run (indicator.run_online) keeps the misses low once the flow has moved
+  100 steps past the training window, where the fixed network misses many more.
 """
 import numpy as np
 import pytest
@@ -212,3 +214,36 @@ def test_learned_indicator_gain():
     assert err[0] <= 1e-6 and err[1] <= 1e-5
     assert lim_b < lim_a
     a.close(), b.close()
+
+
+def test_online_retraining_500k():
+    """500k dam break: trained on 40 warm-up steps, then 100 more steps with the gate fixed or
+    retrained every 20 steps on judged data; the judge over the next 10 steps.  Measured: fixed
+    misses 8.1 % of the significant write-backs (largest missed change 7e-3), online 0.7 % (2e-4)."""
+    kw = dict(batch=1024, pos_weight=8.0, recall=0.9999, margin=16.0)
+    mesh, prm = W.dambreak(scale=1.0)
+    banner("online retraining, dam break", mesh.n_elements)
+    a = Simulation(mesh, prm, RKDG_FP32_BULK)
+    X, y, _ = IND.collect(a, 40, every=1, neg_keep=0.25, seed=1)
+    w, thr, _ = IND.train(X, y, epochs=4, **kw)
+    st = a.state()
+    a.close()
+    out = {}
+    for mode in ("fixed", "online"):
+        s = Simulation(mesh, prm, RKDG_FP32_BULK)
+        s.set_state(st["q"], st["wet"])
+        if mode == "fixed":
+            s.set_indicator(w, threshold=thr)
+            s.step(100)
+            w2, thr2, rounds = w, thr, []
+        else:
+            w2, thr2, _, _, rounds = IND.run_online(s, w, thr, 100, X=X, y=y, every=20, window=1, epochs=2, **kw)
+        s.set_indicator(w2, threshold=thr2, judge=True)
+        s.step(10)
+        j = s.indicator_stats()
+        s.close()
+        out[mode] = (j["missed_significant"] / max(1, j["significant"]), j["max_missed_change"], len(rounds))
+        banner(mode, out[mode])
+    assert out["online"][2] == 4 and out["fixed"][2] == 0
+    assert out["online"][0] < 0.25 * out["fixed"][0] and out["online"][0] < 0.02
+    assert out["online"][1] < out["fixed"][1]
