@@ -224,12 +224,13 @@ def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5, drift_s
     for sim in (a, r):
         sim.close()
     # drift: `drift_steps` more steps from here with the gate fixed, and with online retraining
-    # (IND.run_online: every 20 steps one captured step, labelled by the judge, and 2 epochs from
-    # the current weights), each judged over the `judge_steps` after them
+    # (IND.run_online: one captured step labelled by the judge, then 2 epochs from the current
+    # weights; every 20 steps, or checked every 5 and retrained only when the gate missed over 1 % of
+    # the captured positives), each judged over the `judge_steps` after them
     st = b.state()
     b.close()
     drift = {"steps": drift_steps}
-    for mode in ("fixed", "online"):
+    for mode in ("fixed", "online", "triggered"):
         s = Simulation(mesh, prm, RKDG_FP32_BULK)
         s.set_state(st["q"], st["wet"])
         t0 = time.perf_counter()
@@ -237,10 +238,11 @@ def learned_bench(steps, scale, warm=40, judge_steps=10, kernel_steps=5, drift_s
             s.set_indicator(w, threshold=thr)
             s.step(drift_steps)
             w2, thr2, rounds = w, thr, []
-        else:
-            w2, thr2, X, y, rounds = IND.run_online(s, w, thr, drift_steps, X=X, y=y, every=20, window=1, epochs=2,
-                                                    keep=keep, batch=1024, pos_weight=8.0, recall=0.9999,
-                                                    margin=16.0)
+        else:  # every 20 steps, or checked every 5 and retrained only above 1 % misses
+            every, above = (20, None) if mode == "online" else (5, 0.01)
+            w2, thr2, _, _, rounds = IND.run_online(s, w, thr, drift_steps, X=X, y=y, every=every, window=1,
+                                                    epochs=2, keep=keep, retrain_above=above, batch=1024,
+                                                    pos_weight=8.0, recall=0.9999, margin=16.0)
         s.sync()
         wall = time.perf_counter() - t0
         s.set_indicator(w2, threshold=thr2, judge=True)

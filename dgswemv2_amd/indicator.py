@@ -35,30 +35,38 @@ def init_weights(seed=5):
     return np.concatenate(parts).astype(np.float32)
 
 
-def collect(sim, steps, every=1, neg_keep=1.0, seed=0):
+def collect(sim, steps, every=1, neg_keep=1.0, seed=0, gate_misses=None):
     """Steps `sim` (plain limiter) `steps` full steps with capture on and keeps the samples of
     every `every`-th limiter stage where the limiter may act; negatives are kept with
-    probability neg_keep.  Returns (X [36][n] float32, y [n] float32, stages sampled)."""
+    probability neg_keep.  With an indicator set, `gate_misses` (a threshold) also counts the
+    positives whose logit does not exceed it.  Returns (X [36][n] float32, y [n] float32, stages
+    sampled), plus (misses, positives) when gate_misses is given."""
     rng = np.random.default_rng(seed)
     ns = sim.params.nstages
     sim.capture(True)
     Xs, ys, k = [], [], 0
+    missed = positives = 0
     for _ in range(steps * ns):
         sim.stage()
         k += 1
         if k % every:
             continue
-        c = sim.get_capture()
+        c = sim.get_capture(logits=gate_misses is not None)
         m = c["may_act"] != 0
         y = c["label"][m].astype(np.float32)
         X = c["features"][:, m]
+        if gate_misses is not None:
+            pos = y > 0.5
+            positives += int(pos.sum())
+            missed += int((c["logit"][m][pos] <= gate_misses).sum())
         if neg_keep < 1.0:
             keep = (y > 0) | (rng.uniform(size=y.size) < neg_keep)
             X, y = X[:, keep], y[keep]
         Xs.append(X)
         ys.append(y)
     sim.capture(False)
-    return np.concatenate(Xs, axis=1), np.concatenate(ys), k // every
+    out = (np.concatenate(Xs, axis=1), np.concatenate(ys), k // every)
+    return out + ((missed, positives),) if gate_misses is not None else out
 
 
 def choose_threshold(logits, y, recall=0.999):
@@ -125,13 +133,15 @@ def refine(make_sim, steps, w, thr, X, y, epochs=3, seed=2, **kw):
 
 
 def run_online(sim, w, thr, steps, X=None, y=None, every=20, window=1, epochs=2, keep=4_000_000, neg_keep=0.25,
-               seed=3, **kw):
+               seed=3, retrain_above=None, **kw):
     """Steps `sim` `steps` steps under the gate (w, thr) and retrains on judged data as it goes:
     after every `every` gated steps, `window` more steps run with capture on (their labels are the
     judge's dry run of the plain limit pass on the states the gate produces), the samples join a
     buffer that starts from (X, y) and keeps the newest `keep`, and the network is retrained from its
-    current weights for `epochs` epochs (train's keyword arguments in **kw).  Returns (weights,
-    threshold, X, y, rounds) with one info dict per retraining round."""
+    current weights for `epochs` epochs (train's keyword arguments in **kw).  With `retrain_above`
+    set, a round retrains only when the gate missed more than that share of the captured window's
+    positives (the samples still join the buffer).  Returns (weights, threshold, X, y, rounds) with
+    one info dict per retraining round."""
     rounds = []
     sim.set_indicator(w, threshold=thr)
     done = 0
@@ -142,10 +152,16 @@ def run_online(sim, w, thr, steps, X=None, y=None, every=20, window=1, epochs=2,
         if done >= steps:
             break
         n = min(window, steps - done)
-        Xd, yd, _ = collect(sim, n, every=1, neg_keep=neg_keep, seed=seed + done)
+        if retrain_above is None:
+            Xd, yd, _ = collect(sim, n, every=1, neg_keep=neg_keep, seed=seed + done)
+        else:
+            Xd, yd, _, (missed, positives) = collect(sim, n, every=1, neg_keep=neg_keep, seed=seed + done,
+                                                     gate_misses=thr)
         done += n
         X = Xd if X is None else np.concatenate([X, Xd], axis=1)[:, -keep:]
         y = yd if y is None else np.concatenate([y, yd])[-keep:]
+        if retrain_above is not None and missed <= retrain_above * max(1, positives):
+            continue
         w, thr, info = train(X, y, epochs=epochs, init=w, seed=seed, **kw)
         info["step"] = done
         info["new_samples"] = int(yd.size)
