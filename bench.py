@@ -71,8 +71,11 @@ def time_form(mesh, prm, precision, steps, warmup):
     sim.step(steps)
     sim.sync()
     wall = time.perf_counter() - t0
-    q = sim.state()["q"]
+    t0 = time.perf_counter()
+    q = sim.state()["q"]  # the whole state back to the host through the C-ABI (PCIe), outside `value`
+    readback_ms = (time.perf_counter() - t0) * 1e3
     rep = kernel_window(sim, steps)
+    rep["state_readback_ms"] = readback_ms
     sim.close()
     return wall, rep, q
 
@@ -384,6 +387,9 @@ def main():
             "stage_bytes_alg": rep["stage_bytes_alg"],
             "achieved_GBs": achieved,
             "roofline_frac": achieved / HBM_PEAK_GBS,
+            # the whole state read back to host memory (rkdg_get_state, PCIe): per step taken
+            # together with one read-back, a step costs ms_per_step + state_readback_ms
+            "state_readback_ms": rep["state_readback_ms"],
         }
         log(name, json.dumps(forms[name]))
     if args.form != "all":
