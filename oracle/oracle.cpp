@@ -15,8 +15,11 @@
 // the five wetting-drying scenarios (test_wetting_drying.cpp), the RK ODE test
 // (test_rk_stepper.cpp), plus invariants (lake at rest, mass conservation, manufactured-
 // solution convergence, limiter invariants), closed forms of one forward-Euler stage for
-// bottom friction and tide / flow forcing (tests/test_oracle_forcing.py), and of the trouble
-// check on a step (tests/test_oracle_trouble.py).  Parity unpinned: whole-run trajectories of
+// bottom friction and tide / flow forcing (tests/test_oracle_forcing.py), of the trouble
+// check on a step (tests/test_oracle_trouble.py), and of the Cockburn-Shu limiting step
+// (tests/test_oracle_limiter.py: R / invR as the flux Jacobian's eigensystem, linear fields kept,
+// a local maximum flattened, a hand-worked element, the TVB bound; each with planted mistakes
+// that must fail).  Parity unpinned: whole-run trajectories of
 // the bowl, tidal and dam-break workloads beyond those invariants, which no reference test or
 // stored output covers and which cannot be compared against the reference binary (it does not
 // build here); see the test headers.
@@ -1341,11 +1344,30 @@ static void wetting_drying_kernel(Sim& S, Element& E) {  // swe_post_wet_dry.hpp
 
 // ---- slope limiter (swe_CS_slope_limiter.hpp, swe_trouble_check.hpp, swe_CS_sl_serial.hpp)
 
+// Planted mistakes, for the tests that pin the limiting step (tests/test_oracle_limiter.py): each
+// check there has a twin run with one of these switched on that must fail.  0: the reference's
+// arithmetic.  Set only by those tests (orc_set_plant); never on in any other use of the oracle.
+enum {
+    PLANT_NONE = 0,
+    PLANT_INVR_SIGN = 1,    // invR(0,1) with the wrong sign
+    PLANT_ALPHA_SWAP = 2,   // alpha columns exchanged in w_delta
+    PLANT_T_SIGN = 3,       // vertex rebuild with T = +1 on the diagonal, -1 off it
+    PLANT_THETA_SWAP = 4,   // theta+ and theta- exchanged
+    PLANT_NU_ONE = 5,       // minmod against |w_delta| instead of |nu w_delta|
+    PLANT_R_TRANSPOSE = 6,  // R transposed in the characteristic back-transform
+    PLANT_NO_THETA = 7,     // positive / negative balance skipped
+    PLANT_TVB_OFF = 8       // |w_tilda| <= M r^2 branch skipped
+};
+static int g_plant = PLANT_NONE;
+
 static void R_mat(double g, double h, double u, double v, double nx, double ny, double R[3][3]) {  // swe_jacobian.hpp:462-480
     double c = std::sqrt(g * h);
     R[0][0] = 1.0, R[0][1] = 0.0, R[0][2] = 1.0;
     R[1][0] = u - c * nx, R[1][1] = -ny, R[1][2] = u + c * nx;
     R[2][0] = v - c * ny, R[2][1] = nx, R[2][2] = v + c * ny;
+    if (g_plant == PLANT_R_TRANSPOSE)
+        for (int i = 0; i < 3; ++i)
+            for (int j = i + 1; j < 3; ++j) std::swap(R[i][j], R[j][i]);
 }
 
 static void invR_mat(double g, double h, double u, double v, double nx, double ny, double I[3][3]) {  // :531-550
@@ -1354,10 +1376,102 @@ static void invR_mat(double g, double h, double u, double v, double nx, double n
     I[0][0] = (c + un) / (2 * c), I[0][1] = -nx / (2 * c), I[0][2] = -ny / (2 * c);
     I[1][0] = u * ny - v * nx, I[1][1] = -ny, I[1][2] = nx;
     I[2][0] = (c - un) / (2 * c), I[2][1] = nx / (2 * c), I[2][2] = ny / (2 * c);
+    if (g_plant == PLANT_INVR_SIGN) I[0][1] = -I[0][1];
 }
 
 static void mv3(const double A[3][3], const double* x, double* y) {
     for (int i = 0; i < 3; ++i) y[i] = A[i][0] * x[0] + A[i][1] * x[1] + A[i][2] * x[2];
+}
+
+// slope_limiting_prepare_element_kernel (swe_CS_slope_limiter.hpp:8-20): barycentre and edge
+// midpoint values of the linear part (vertex values q_lin, var x vertex).
+static void cs_prepare(const Mat& q_lin, double* q_bar, Mat& q_mid) {
+    for (int k = 0; k < 3; ++k)
+        q_bar[k] = q_lin(k, 0) * (1.0 / 3.0) + q_lin(k, 1) * (1.0 / 3.0) + q_lin(k, 2) * (1.0 / 3.0);
+    Mat chi(3, 3);  // chi_midpts (master_triangle.tpp:23-33)
+    chi(0, 0) = 0.0, chi(0, 1) = 0.5, chi(0, 2) = 0.5;
+    chi(1, 0) = 0.5, chi(1, 1) = 0.0, chi(1, 2) = 0.5;
+    chi(2, 0) = 0.5, chi(2, 1) = 0.5, chi(2, 2) = 0.0;
+    q_mid = mul(q_lin, chi);
+}
+
+// The limiting step of one wet element with wet neighbours (slope_limiting_kernel,
+// swe_CS_slope_limiter.hpp:106-187): characteristic decomposition along each median
+// (R / invR at the barycentre state, swe_jacobian.hpp:462-480, 531-550), TVB-modified minmod,
+// positive / negative balance, vertex rebuild q_bar + delta T.  Inputs: q_lin (var x vertex),
+// q_bar, q_mid (var x bound), the neighbours' barycentre values q_nb[bound][var] (own value on
+// boundary faces), the barycentre bathymetry, medians, alpha, r^2.  Outputs: q_vrtx (var x vertex),
+// delta (var x bound, after the balance) and, per variable, whether the reference writes the
+// rebuilt vertex values back (relative change above 1e-12, :181-187).
+static void cs_limit_core(double g, double M, double nu, const Mat& q_lin, const double* q_bar, const Mat& q_mid,
+                          const double (*q_nb)[3], double bath_bar, const std::array<double, 2>* median,
+                          const std::array<double, 2>* alpha, const double* r_sq, Mat& q_vrtx, double delta[3][3],
+                          bool limited[3]) {
+    double h = q_bar[0] + bath_bar;
+    double u = q_bar[1] / h;
+    double v = q_bar[2] / h;
+    for (int b = 0; b < 3; ++b) {
+        int e1 = b, e2 = (b + 1) % 3;
+        double mx = median[b][0], my = median[b][1];
+        double R[3][3], iR[3][3];
+        R_mat(g, h, u, v, mx, my, R);
+        invR_mat(g, h, u, v, mx, my, iR);
+        double qm[3] = {q_mid(0, b), q_mid(1, b), q_mid(2, b)};
+        double wm[3], w0[3], w1[3], w2[3];
+        mv3(iR, qm, wm);
+        mv3(iR, q_bar, w0);
+        mv3(iR, q_nb[e1], w1);
+        mv3(iR, q_nb[e2], w2);
+        const double a0 = g_plant == PLANT_ALPHA_SWAP ? alpha[b][1] : alpha[b][0];
+        const double a1 = g_plant == PLANT_ALPHA_SWAP ? alpha[b][0] : alpha[b][1];
+        const double nuu = g_plant == PLANT_NU_ONE ? 1.0 : nu;
+        double wt[3], wd[3], dc[3];
+        for (int k = 0; k < 3; ++k) {
+            wt[k] = wm[k] - w0[k];
+            wd[k] = a0 * (w1[k] - w0[k]) + a1 * (w2[k] - w0[k]);
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (g_plant != PLANT_TVB_OFF && std::abs(wt[k]) <= M * r_sq[b])
+                dc[k] = wt[k];
+            else if (std::signbit(wt[k]) == std::signbit(wd[k]))
+                dc[k] = std::copysign(1.0, wt[k]) * std::min(std::abs(wt[k]), std::abs(nuu * wd[k]));
+            else
+                dc[k] = 0.0;
+        }
+        double col[3];
+        mv3(R, dc, col);
+        for (int k = 0; k < 3; ++k) delta[k][b] = col[k];
+    }
+    for (int k = 0; k < 3; ++k) {
+        double sum = 0.0;
+        for (int b = 0; b < 3; ++b) sum += delta[k][b];
+        if (sum != 0.0 && g_plant != PLANT_NO_THETA) {
+            double pos = 0.0, neg = 0.0;
+            for (int b = 0; b < 3; ++b) {
+                pos += std::max(0.0, delta[k][b]);
+                neg += std::max(0.0, -delta[k][b]);
+            }
+            double tp = std::min(1.0, neg / pos);
+            double tn = std::min(1.0, pos / neg);
+            if (g_plant == PLANT_THETA_SWAP) std::swap(tp, tn);
+            for (int b = 0; b < 3; ++b)
+                delta[k][b] = tp * std::max(0.0, delta[k][b]) - tn * std::max(0.0, -delta[k][b]);
+        }
+    }
+    const double sgn = g_plant == PLANT_T_SIGN ? -1.0 : 1.0;
+    const double T[3][3] = {{-sgn, sgn, sgn}, {sgn, -sgn, sgn}, {sgn, sgn, -sgn}};
+    for (int k = 0; k < 3; ++k)
+        for (int vv = 0; vv < 3; ++vv) {
+            double s = delta[k][0] * T[0][vv] + delta[k][1] * T[1][vv] + delta[k][2] * T[2][vv];
+            q_vrtx(k, vv) = q_bar[k] + s;
+        }
+    for (int k = 0; k < 3; ++k) {
+        double d0 = q_vrtx(k, 0) - q_lin(k, 0), d1 = q_vrtx(k, 1) - q_lin(k, 1), d2 = q_vrtx(k, 2) - q_lin(k, 2);
+        double del_norm = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+        double l0 = q_lin(k, 0), l1 = q_lin(k, 1), l2 = q_lin(k, 2);
+        double q_norm = std::sqrt(l0 * l0 + l1 * l1 + l2 * l2);
+        limited[k] = del_norm / q_norm > 1.0e-12;
+    }
 }
 
 static void slope_limiter(Sim& S) {
@@ -1367,15 +1481,8 @@ static void slope_limiter(Sim& S) {
     for (auto& E : S.elts) {
         if (!E.wet) continue;
         E.sl_q_lin = mul(E.q[st], m.T_bl);
-        for (int k = 0; k < 3; ++k)
-            E.q_at_baryctr[k] = E.sl_q_lin(k, 0) * (1.0 / 3.0) + E.sl_q_lin(k, 1) * (1.0 / 3.0) +
-                                E.sl_q_lin(k, 2) * (1.0 / 3.0);
+        cs_prepare(E.sl_q_lin, E.q_at_baryctr, E.sl_q_at_midpts);
         E.sl_q_at_vrtx = E.sl_q_lin;
-        Mat chi(3, 3);  // chi_midpts (master_triangle.tpp:23-33)
-        chi(0, 0) = 0.0, chi(0, 1) = 0.5, chi(0, 2) = 0.5;
-        chi(1, 0) = 0.5, chi(1, 1) = 0.0, chi(1, 2) = 0.5;
-        chi(2, 0) = 0.5, chi(2, 1) = 0.5, chi(2, 2) = 0.0;
-        E.sl_q_at_midpts = mul(E.sl_q_lin, chi);
     }
     // prepare interface (:22-35)
     for (auto& I : S.intfaces) {
@@ -1458,73 +1565,14 @@ static void slope_limiter(Sim& S) {
         E.limited[0] = E.limited[1] = E.limited[2] = false;
         if (!(E.wet && E.wet_neigh[0] && E.wet_neigh[1] && E.wet_neigh[2])) continue;
         Mat& q = E.q[st];
-        double h = E.q_at_baryctr[0] + E.bath_at_baryctr;
-        double u = E.q_at_baryctr[1] / h;
-        double v = E.q_at_baryctr[2] / h;
-        double delta[3][3];  // var x bound
-        for (int b = 0; b < 3; ++b) {
-            int e1 = b, e2 = (b + 1) % 3;
-            double mx = E.median[b][0], my = E.median[b][1];
-            double R[3][3], iR[3][3];
-            R_mat(g, h, u, v, mx, my, R);
-            invR_mat(g, h, u, v, mx, my, iR);
-            double qm[3] = {E.sl_q_at_midpts(0, b), E.sl_q_at_midpts(1, b), E.sl_q_at_midpts(2, b)};
-            double wm[3], w0[3], w1[3], w2[3];
-            mv3(iR, qm, wm);
-            mv3(iR, E.q_at_baryctr, w0);
-            mv3(iR, E.q_at_baryctr_neigh[e1], w1);
-            mv3(iR, E.q_at_baryctr_neigh[e2], w2);
-            double wt[3], wd[3], dc[3];
-            for (int k = 0; k < 3; ++k) {
-                wt[k] = wm[k] - w0[k];
-                wd[k] = E.alpha[b][0] * (w1[k] - w0[k]) + E.alpha[b][1] * (w2[k] - w0[k]);
-            }
-            for (int k = 0; k < 3; ++k) {
-                if (std::abs(wt[k]) <= S.M * E.r_sq[b])
-                    dc[k] = wt[k];
-                else if (std::signbit(wt[k]) == std::signbit(wd[k]))
-                    dc[k] = std::copysign(1.0, wt[k]) * std::min(std::abs(wt[k]), std::abs(S.nu * wd[k]));
-                else
-                    dc[k] = 0.0;
-            }
-            double col[3];
-            mv3(R, dc, col);
-            for (int k = 0; k < 3; ++k) delta[k][b] = col[k];
-        }
-        for (int k = 0; k < 3; ++k) {
-            double sum = 0.0;
-            for (int b = 0; b < 3; ++b) sum += delta[k][b];
-            if (sum != 0.0) {
-                double pos = 0.0, neg = 0.0;
-                for (int b = 0; b < 3; ++b) {
-                    pos += std::max(0.0, delta[k][b]);
-                    neg += std::max(0.0, -delta[k][b]);
-                }
-                double tp = std::min(1.0, neg / pos);
-                double tn = std::min(1.0, pos / neg);
-                for (int b = 0; b < 3; ++b)
-                    delta[k][b] = tp * std::max(0.0, delta[k][b]) - tn * std::max(0.0, -delta[k][b]);
-            }
-        }
-        const double T[3][3] = {{-1.0, 1.0, 1.0}, {1.0, -1.0, 1.0}, {1.0, 1.0, -1.0}};
+        double delta[3][3];
+        cs_limit_core(g, S.M, S.nu, E.sl_q_lin, E.q_at_baryctr, E.sl_q_at_midpts, E.q_at_baryctr_neigh,
+                      E.bath_at_baryctr, E.median, E.alpha, E.r_sq, E.sl_q_at_vrtx, delta, E.limited);
         for (int k = 0; k < 3; ++k)
-            for (int vv = 0; vv < 3; ++vv) {
-                double s = delta[k][0] * T[0][vv] + delta[k][1] * T[1][vv] + delta[k][2] * T[2][vv];
-                E.sl_q_at_vrtx(k, vv) = E.q_at_baryctr[k] + s;
-            }
-        for (int k = 0; k < 3; ++k) {
-            double d0 = E.sl_q_at_vrtx(k, 0) - E.sl_q_lin(k, 0), d1 = E.sl_q_at_vrtx(k, 1) - E.sl_q_lin(k, 1),
-                   d2 = E.sl_q_at_vrtx(k, 2) - E.sl_q_lin(k, 2);
-            double del_norm = std::sqrt(d0 * d0 + d1 * d1 + d2 * d2);
-            double l0 = E.sl_q_lin(k, 0), l1 = E.sl_q_lin(k, 1), l2 = E.sl_q_lin(k, 2);
-            double q_norm = std::sqrt(l0 * l0 + l1 * l1 + l2 * l2);
-            if (del_norm / q_norm > 1.0e-12) {
-                E.limited[k] = true;
+            if (E.limited[k])
                 for (unsigned d = 0; d < m.ndof; ++d)
                     q(k, d) = E.sl_q_at_vrtx(k, 0) * m.T_lb(0, d) + E.sl_q_at_vrtx(k, 1) * m.T_lb(1, d) +
                               E.sl_q_at_vrtx(k, 2) * m.T_lb(2, d);
-            }
-        }
     }
 }
 
@@ -1700,6 +1748,65 @@ void orc_get_time(void* h, double* t, int* step, int* stage) {
 }
 
 double orc_l2_residual(void* h) { return orc::l2_residual(*(orc::Sim*)h); }
+
+// ---- the limiting step in isolation (tests/test_oracle_limiter.py)
+// Test-only switch of a planted mistake in the limiting step (orc::PLANT_*; 0 = none).
+void orc_set_plant(int kind) { orc::g_plant = kind; }
+// R and invR at (h, u, v) along the unit direction (nx, ny), row-major 3x3.
+void orc_R(double g, double h, double u, double v, double nx, double ny, double* R) {
+    double M[3][3];
+    orc::R_mat(g, h, u, v, nx, ny, M);
+    for (int i = 0; i < 9; ++i) R[i] = M[i / 3][i % 3];
+}
+void orc_invR(double g, double h, double u, double v, double nx, double ny, double* I) {
+    double M[3][3];
+    orc::invR_mat(g, h, u, v, nx, ny, M);
+    for (int i = 0; i < 9; ++i) I[i] = M[i / 3][i % 3];
+}
+// One element's limiting step from its linear vertex values q_lin [var][vertex] (barycentre and
+// midpoints formed as the prepare kernel forms them), the neighbours' barycentre values
+// q_nb [bound][var], the barycentre bathymetry, medians [bound][2], alpha [bound][2] and r^2 [bound].
+// Writes the rebuilt vertex values q_vrtx [var][vertex], delta [var][bound] after the balance,
+// and limited [var] (1 where the reference writes back).
+void orc_cs_limit_element(double g, double M, double nu, const double* q_lin, const double* q_nb, double bath_bar,
+                          const double* median, const double* alpha, const double* r_sq, double* q_vrtx, double* delta,
+                          int* limited) {
+    orc::Mat ql(3, 3), qm, qv(3, 3);
+    for (int i = 0; i < 9; ++i) ql(i / 3, i % 3) = q_lin[i];
+    double qb[3];
+    orc::cs_prepare(ql, qb, qm);
+    double nb[3][3];
+    std::array<double, 2> med[3], al[3];
+    double rs[3];
+    for (int b = 0; b < 3; ++b) {
+        for (int k = 0; k < 3; ++k) nb[b][k] = q_nb[3 * b + k];
+        med[b] = {median[2 * b], median[2 * b + 1]};
+        al[b] = {alpha[2 * b], alpha[2 * b + 1]};
+        rs[b] = r_sq[b];
+    }
+    double dl[3][3];
+    bool lim[3];
+    orc::cs_limit_core(g, M, nu, ql, qb, qm, nb, bath_bar, med, al, rs, qv, dl, lim);
+    for (int i = 0; i < 9; ++i) q_vrtx[i] = qv(i / 3, i % 3), delta[i] = dl[i / 3][i % 3];
+    for (int k = 0; k < 3; ++k) limited[k] = lim[k] ? 1 : 0;
+}
+// Runs the whole slope limiter (prepare, trouble check, limiting step) once on the current state,
+// as a stage does after wetting-drying; only with slope limiting on.
+int orc_apply_limiter(void* h) {
+    orc::Sim& S = *(orc::Sim*)h;
+    if (!S.prm.slope_limiting) return RKDG_ERR_ARG;
+    orc::slope_limiter(S);
+    return RKDG_OK;
+}
+// Limiter geometry of one element: medians [3][2], alpha [3][2], r^2 [3] (swe_pre_init_data.hpp:252-326).
+void orc_limiter_element_geometry(void* h, int e, double* median, double* alpha, double* r_sq) {
+    const orc::Element& E = ((orc::Sim*)h)->elts[e];
+    for (int b = 0; b < 3; ++b) {
+        median[2 * b] = E.median[b][0], median[2 * b + 1] = E.median[b][1];
+        alpha[2 * b] = E.alpha[b][0], alpha[2 * b + 1] = E.alpha[b][1];
+        r_sq[b] = E.r_sq[b];
+    }
+}
 
 int orc_num_interfaces(void* h) { return (int)((orc::Sim*)h)->intfaces.size(); }
 int orc_num_boundaries(void* h) { return (int)((orc::Sim*)h)->bounds.size(); }

@@ -49,6 +49,12 @@ def load():
             "orc_element_integrals": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp]),
             "orc_element_gp_global": (C.c_int, [C.c_int, dp, dp, dp]),
             "orc_wet_dry_single": (C.c_int, [dp, dp, dp, C.c_double, C.c_double, C.c_int, dp, ip, ip, dp]),
+            "orc_set_plant": (None, [C.c_int]),
+            "orc_R": (None, [C.c_double] * 6 + [dp]),
+            "orc_invR": (None, [C.c_double] * 6 + [dp]),
+            "orc_cs_limit_element": (None, [C.c_double] * 3 + [dp, dp, C.c_double, dp, dp, dp, dp, dp, ip]),
+            "orc_apply_limiter": (C.c_int, [C.c_void_p]),
+            "orc_limiter_element_geometry": (None, [C.c_void_p, C.c_int, dp, dp, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(lib, name)
