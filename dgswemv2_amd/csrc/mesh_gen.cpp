@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <new>
+#include <memory>
 #include <vector>
 
 #include "../../include/rkdg.h"
@@ -47,8 +48,9 @@ struct Fourier {
 
 extern "C" rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* sp, rkdg_mesh** out, rkdg_mesh_desc* desc) {
     if (!sp || !out || !desc || sp->m < 1 || sp->n < 1 || sp->pattern < 0 || sp->pattern > 2) return RKDG_ERR_ARG;
-    rkdg_mesh* M = new (std::nothrow) rkdg_mesh();
-    if (!M) return RKDG_ERR_ARG;
+    return rkdg::guarded("rkdg_mesh_generate", [&]() -> rkdg_status {
+    std::unique_ptr<rkdg_mesh> owner(new rkdg_mesh());
+    rkdg_mesh* M = owner.get();
     const int m = sp->m, n = sp->n;
     const double p1[2] = {sp->x1, sp->y1}, p2[2] = {sp->x2, sp->y1}, p3[2] = {sp->x2, sp->y2}, p4[2] = {sp->x1, sp->y2};
     const double dxi = 2.0 / m, deta = 2.0 / n;
@@ -165,8 +167,9 @@ extern "C" rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* sp, rkdg_me
 
     *desc = rkdg_mesh_desc{};
     M->fill(desc);
-    *out = M;
+    *out = owner.release();
     return RKDG_OK;
+    });
 }
 
 extern "C" void rkdg_mesh_free(rkdg_mesh* mesh) { delete mesh; }

@@ -49,13 +49,24 @@ std::istringstream next_line(std::istream& is, const char* what) {
     return std::istringstream(line);
 }
 
+// A count read from a line of its own (an ADCIRC count field): missing, unparsable or negative
+// counts are file errors, not sizes.
+long read_count(std::istringstream ss, const char* what) {
+    long n = -1;
+    if (!(ss >> n) || n < 0) throw ReadError{std::string("bad ") + what + " count"};
+    return n;
+}
+
 Fort14 read_fort14(const std::string& path) {
     std::ifstream ifs(path);
     if (!ifs) throw ReadError{"ADCIRC mesh file " + path + " was not found"};
     Fort14 F;
     std::getline(ifs, F.name);
     long ne = 0, nn = 0;
-    next_line(ifs, "element/node counts") >> ne >> nn;
+    {
+        auto ss = next_line(ifs, "element/node counts");
+        if (!(ss >> ne >> nn) || ne < 0 || nn < 0) throw ReadError{"bad element/node counts"};
+    }
     if (ne <= 0 || nn <= 0) throw ReadError{"bad element/node counts"};
     for (long i = 0; i < nn; ++i) {
         long id;
@@ -74,21 +85,23 @@ Fort14 read_fort14(const std::string& path) {
         if (k != 3) throw ReadError{"only triangles are supported"};
         if (!F.elements.emplace(id, v).second) throw ReadError{"element defined twice"};
     }
-    long nope = 0, n;
-    next_line(ifs, "open boundaries") >> nope;
+    long n;
+    const long nope = read_count(next_line(ifs, "open boundaries"), "open boundary");
     next_line(ifs, "open boundary nodes");
     for (long b = 0; b < nope; ++b) {
-        next_line(ifs, "open boundary") >> n;
+        n = read_count(next_line(ifs, "open boundary"), "open boundary node");
         std::vector<long> v(n);
         for (long i = 0; i < n; ++i) next_line(ifs, "open boundary node") >> v[i];
         F.open.push_back(std::move(v));
     }
-    long nbou = 0;
-    next_line(ifs, "land boundaries") >> nbou;
+    const long nbou = read_count(next_line(ifs, "land boundaries"), "land boundary");
     next_line(ifs, "land boundary nodes");
     for (long b = 0; b < nbou; ++b) {
         Segment S;
-        next_line(ifs, "boundary segment") >> n >> S.ibtype;
+        {
+            auto ss = next_line(ifs, "boundary segment");
+            if (!(ss >> n >> S.ibtype) || n < 0) throw ReadError{"bad boundary segment node count"};
+        }
         const int t = S.ibtype % 10;
         if (t == 0 || t == 1 || t == 2 || S.ibtype == 77 || S.ibtype == 88) {
             S.nodes.resize(n);
@@ -107,11 +120,10 @@ Fort14 read_fort14(const std::string& path) {
     }
     std::string line;
     if (std::getline(ifs, line) && line.find_first_not_of(" \t\r") != std::string::npos) {  // generic boundaries
-        long ngen = 0;
-        std::istringstream(line) >> ngen;
+        const long ngen = read_count(std::istringstream(line), "generic boundary");
         next_line(ifs, "generic boundary nodes");
         for (long b = 0; b < ngen; ++b) {
-            next_line(ifs, "generic boundary") >> n;
+            n = read_count(next_line(ifs, "generic boundary"), "generic boundary node");
             std::vector<long> v(n);
             for (long i = 0; i < n; ++i) next_line(ifs, "generic boundary node") >> v[i];
             F.generic.push_back(std::move(v));
@@ -191,8 +203,8 @@ Bcis read_bcis(const char* path, NodeFn&& node) {
         long btype = -1, nnodes = 0;
         std::istringstream(line) >> btype >> nnodes;
         if (btype == RKDG_BC_TIDE || btype == RKDG_BC_FLOW) {
-            long ncon = 0;
-            next_line(ifs, "constituent count") >> ncon;
+            if (nnodes < 0) throw ReadError{"bad forcing node count"};
+            const long ncon = read_count(next_line(ifs, "constituent count"), "constituent");
             std::vector<std::array<double, 3>> here;
             std::map<long, std::vector<std::array<double, 2>>> block;
             for (long c = 0; c < ncon; ++c) {
@@ -323,6 +335,8 @@ extern "C" rkdg_status rkdg_mesh_read_adcirc(const char* fort14_path, const char
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");
+    } catch (const std::exception& ex) {
+        rkdg::set_create_error(std::string("mesh file error: ") + ex.what());
     }
     return RKDG_ERR_ARG;
 }
@@ -518,6 +532,8 @@ extern "C" rkdg_status rkdg_mesh_write_adcirc(const rkdg_mesh_desc* d, const cha
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");
+    } catch (const std::exception& ex) {
+        rkdg::set_create_error(std::string("mesh file error: ") + ex.what());
     }
     return RKDG_ERR_ARG;
 }
@@ -610,6 +626,8 @@ extern "C" rkdg_status rkdg_mesh_write_meta(const rkdg_mesh_desc* d, const char*
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");
+    } catch (const std::exception& ex) {
+        rkdg::set_create_error(std::string("mesh file error: ") + ex.what());
     }
     return RKDG_ERR_ARG;
 }
@@ -714,6 +732,8 @@ extern "C" rkdg_status rkdg_mesh_read_meta(const char* meta_path, const char* bc
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");
+    } catch (const std::exception& ex) {
+        rkdg::set_create_error(std::string("mesh file error: ") + ex.what());
     }
     return RKDG_ERR_ARG;
 }

@@ -2,6 +2,8 @@
 #pragma once
 
 #include <cstdint>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -9,7 +11,23 @@
 
 namespace rkdg {
 void set_create_error(const std::string& msg);  // the message rkdg_create_error() returns
+
+// Runs the body of an extern "C" host entry point: no exception crosses the C-ABI; any that
+// escapes the body becomes RKDG_ERR_ARG with its message in rkdg_create_error().
+template <typename F>
+rkdg_status guarded(const char* what, F&& body) {
+    try {
+        return body();
+    } catch (const std::bad_alloc&) {
+        set_create_error(std::string(what) + ": out of host memory");
+    } catch (const std::exception& ex) {
+        set_create_error(std::string(what) + ": " + ex.what());
+    } catch (...) {
+        set_create_error(std::string(what) + ": unknown error");
+    }
+    return RKDG_ERR_ARG;
 }
+}  // namespace rkdg
 
 struct rkdg_mesh {
     std::vector<double> xyz;

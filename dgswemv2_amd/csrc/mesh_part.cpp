@@ -59,6 +59,7 @@ void rcb(std::vector<int>& idx, size_t lo, size_t hi, int p0, int np, const std:
 }  // namespace
 
 extern "C" rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* d, int32_t nparts, int32_t* part) {
+    return rkdg::guarded("rkdg_partition_rcb", [&]() -> rkdg_status {
     if (!d || !part || nparts < 1 || d->n_elements < nparts || !d->node_xyz || !d->elt_nodes) {
         rkdg::set_create_error("bad arguments");
         return RKDG_ERR_ARG;
@@ -81,6 +82,7 @@ extern "C" rkdg_status rkdg_partition_rcb(const rkdg_mesh_desc* d, int32_t npart
     std::iota(idx.begin(), idx.end(), 0);
     rcb(idx, 0, ne, 0, nparts, cx, cy, part);
     return RKDG_OK;
+    });
 }
 
 extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part, int32_t p, rkdg_mesh** out,
@@ -198,6 +200,8 @@ extern "C" rkdg_status rkdg_submesh(const rkdg_mesh_desc* d, const int32_t* part
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");
+    } catch (const std::exception& ex) {
+        rkdg::set_create_error(std::string("partition error: ") + ex.what());
     }
     return RKDG_ERR_ARG;
 }
@@ -300,6 +304,8 @@ extern "C" rkdg_status rkdg_mesh_renumber(const rkdg_mesh_desc* d, rkdg_mesh** o
         rkdg::set_create_error(err.msg);
     } catch (const std::bad_alloc&) {
         rkdg::set_create_error("out of host memory");
+    } catch (const std::exception& ex) {
+        rkdg::set_create_error(std::string("partition error: ") + ex.what());
     }
     return RKDG_ERR_ARG;
 }

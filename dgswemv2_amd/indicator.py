@@ -70,12 +70,30 @@ def collect(sim, steps, every=1, neg_keep=1.0, seed=0, gate_misses=None):
 
 
 def choose_threshold(logits, y, recall=0.999):
-    """Largest threshold whose gate (logit > threshold) opens for at least `recall` of the positives."""
+    """Largest threshold whose gate (logit > threshold) opens for at least `recall` of the positives;
+    None when y holds no positive (no threshold is defined by them)."""
     pos = np.sort(np.asarray(logits)[np.asarray(y) > 0.5])
     if pos.size == 0:
-        return float("inf")
+        return None
     k = int(np.floor((1.0 - recall) * pos.size))
     return float(np.nextafter(np.float32(pos[k]), np.float32(-np.inf)))
+
+
+def pick_threshold(z_val, y_val, z_train_pos, recall, margin):
+    """Gate threshold from held-out logits z_val / labels y_val, lowered by `margin`.  With no
+    held-out positive it falls back to the training positives' logits z_train_pos, and with no
+    positive at all to -inf (gate open wherever the limiter may act: the plain limiter's decisions),
+    warning in both cases.  Returns (threshold, source) with source "holdout", "training" or "none"."""
+    import warnings
+    thr = choose_threshold(z_val, y_val, recall)
+    if thr is not None:
+        return thr - margin, "holdout"
+    z_train_pos = np.asarray(z_train_pos)
+    if z_train_pos.size:
+        warnings.warn("no positive sample in the held-out split: threshold from the training positives")
+        return choose_threshold(z_train_pos, np.ones(z_train_pos.size), recall) - margin, "training"
+    warnings.warn("no positive sample at all: the gate stays open wherever the limiter may act")
+    return float("-inf"), "none"
 
 
 def train(X, y, epochs=6, batch=1024, lr=2e-3, pos_weight=8.0, recall=0.9999, margin=16.0, holdout=0.1, seed=5,
@@ -99,14 +117,16 @@ def train(X, y, epochs=6, batch=1024, lr=2e-3, pos_weight=8.0, recall=0.9999, ma
         losses = t.train(epochs, batch=batch, lr=lr, pos_weight=pos_weight, seed=seed)
         w = t.weights()
         z = t.forward(np.ascontiguousarray(X[:, va]))
+        yv = y[va]
+        tp = tr[y[tr] > 0.5] if not (yv > 0.5).any() else tr[:0]
+        z_tp = t.forward(np.ascontiguousarray(X[:, tp])) if tp.size else np.zeros(0, np.float32)
     finally:
         t.close()
-    yv = y[va]
-    thr = choose_threshold(z, yv, recall) - margin
+    thr, source = pick_threshold(z, yv, z_tp, recall, margin)
     gate = z > thr
     info = {"samples": int(n), "positives": float(y.mean()), "epoch_loss": [float(v) for v in losses],
-            "val_recall": float(gate[yv > 0.5].mean()) if (yv > 0.5).any() else 1.0,
-            "val_open": float(gate.mean()), "threshold": thr}
+            "val_recall": float(gate[yv > 0.5].mean()) if (yv > 0.5).any() else None,
+            "val_open": float(gate.mean()), "threshold": thr, "threshold_from": source}
     return w, thr, info
 
 

@@ -181,3 +181,42 @@ def test_weir_meta_round_trip_and_oracle(tmp_path):
                     b"3 = number of nodes\n0 0 0 1\n1 1 0 1\n2 0 1 1\n")
     with pytest.raises(RkdgError, match="undefined boundary type 7"):
         read_meta(bad)
+
+
+ONE_TRIANGLE = """one
+1 3
+1 0.0 0.0 5.0
+2 1.0 0.0 5.0
+3 0.0 1.0 5.0
+1 3 1 2 3
+{nope} = NOPE
+{neta} = NETA
+{open}0 = NBOU
+0 = NVEL
+"""
+
+
+@pytest.mark.parametrize("nope,neta,body,needle", [
+    ("1", "-5", "-5 = NVDLL\n", "open boundary node"),
+    ("-1", "0", "", "open boundary"),
+    ("x", "0", "", "open boundary"),
+])
+def test_malformed_counts_are_errors(tmp_path, nope, neta, body, needle):
+    """Negative or unparsable boundary counts in a .14 are reported as RKDG_ERR_ARG with a message
+    (the host process must not abort on a bad file)."""
+    f = tmp_path / "bad.14"
+    f.write_text(ONE_TRIANGLE.format(nope=nope, neta=neta, open=body))
+    with pytest.raises(RkdgError) as ei:
+        read_adcirc(f)
+    assert ei.value.status == 1 and needle in str(ei.value)
+
+
+def test_malformed_segment_and_element_counts(tmp_path):
+    seg = ONE_TRIANGLE.format(nope="0", neta="0", open="").replace("0 = NBOU\n0 = NVEL\n",
+                                                                     "1 = NBOU\n3 = NVEL\n-3 0\n")
+    (tmp_path / "seg.14").write_text(seg)
+    with pytest.raises(RkdgError, match="boundary segment"):
+        read_adcirc(tmp_path / "seg.14")
+    (tmp_path / "ne.14").write_text("neg\n-1 3\n")
+    with pytest.raises(RkdgError, match="element/node counts"):
+        read_adcirc(tmp_path / "ne.14")

@@ -192,3 +192,24 @@ def test_labels_are_significant_write_backs():
     # where nothing was written back the state is unchanged
     untouched = raw == 0
     assert np.array_equal(st["q"][:, :, untouched], q[:, :, untouched])
+
+
+def test_threshold_choice_and_fallbacks():
+    """The gate threshold: `recall` of the held-out positives open the gate; with no held-out
+    positive the training positives decide; with none at all the gate opens everywhere (-inf), and
+    never silently closes (a +inf threshold would switch the limiter off)."""
+    import warnings
+    from dgswemv2_amd.indicator import choose_threshold, pick_threshold
+    z = np.array([-3.0, 0.5, 2.0, 4.0, -1.0], np.float32)
+    y = np.array([0, 1, 1, 1, 0], np.float32)
+    t = choose_threshold(z, y, recall=1.0)
+    assert t < 0.5 and np.all((z > t)[y > 0.5])
+    assert choose_threshold(z, np.zeros(5), recall=0.99) is None
+    thr, src = pick_threshold(z, y, np.zeros(0), 1.0, 2.0)
+    assert src == "holdout" and thr == pytest.approx(t - 2.0)
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        thr, src = pick_threshold(z, np.zeros(5), np.array([1.5, 3.0], np.float32), 1.0, 2.0)
+        assert src == "training" and thr < 1.5 - 2.0 + 1e-6 and len(w) == 1
+        thr, src = pick_threshold(z, np.zeros(5), np.zeros(0), 1.0, 2.0)
+        assert src == "none" and thr == float("-inf") and np.all(z > thr)
