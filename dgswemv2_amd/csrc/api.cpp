@@ -52,6 +52,11 @@ struct EngineBase {
     virtual rkdg_status inst_set(int i, const double* q, const uint8_t* wet) = 0;
     virtual rkdg_status inst_time(int i, double* t, int* step, int* stage) = 0;
     virtual rkdg_status snapshot(bool restore) = 0;  // save / restore the step-boundary state
+    // ensembles whose instances ramp their forcing: the per-stage ramps of the next nsteps steps
+    // (nsteps < kRampRows), computed on the host and queued for the device stepper
+    static constexpr int kRampRows = 128;
+    virtual rkdg_status ramp_prepare(int nsteps) = 0;
+    virtual bool ramps() const = 0;
     // learned troubled-cell indicator
     virtual rkdg_status set_indicator(const rkdg_mlp_weights* w, int judge) = 0;
     virtual rkdg_status indicator_stats(rkdg_indicator_stats* st, bool reset) = 0;
@@ -96,6 +101,12 @@ struct Engine final : EngineBase {
     double *inst_tr_d = nullptr, *inst_t_d = nullptr, *inst_dt_d = nullptr, *inst_rd_d = nullptr, *drk_d = nullptr;
     Real* coef_d = nullptr;
     unsigned long long* step_ctr_d = nullptr;
+    // ramped ensembles: ramp of each instance at each stage of kRampRows steps from ramp_base,
+    // [row][stage][instance], filled on the host (pinned staging copy) by ramp_prepare
+    bool ramp_on = false;
+    double* ramp_tab_d = nullptr;
+    long long* ramp_base_d = nullptr;
+    double* ramp_tab_h = nullptr;  // kRampRows * ns * n values, then the base step
     std::vector<void*> snap;  // snapshot buffers (autotuning)
     // learned indicator: packed bf16 weights, vectors, judge counters, capture buffers
     bool ind_on = false, ind_judge = false, cap_on = false, cap_valid = false, cap_logits = false;
@@ -174,6 +185,7 @@ struct Engine final : EngineBase {
         for (auto& pr : lim_ev) hipEventDestroy(pr.first), hipEventDestroy(pr.second);
         for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
         for (void* p : allocs) hipFree(p);
+        if (ramp_tab_h) hipHostFree(ramp_tab_h);
     }
 
     bool fail(const char* what, hipError_t e) {
@@ -280,6 +292,14 @@ struct Engine final : EngineBase {
             !upload(inst_step_d, zero_i) || !upload(inst_stage_d, zero_i) || !upload(inst_tr_d, tr) ||
             !upload(coef_d, coef) || !alloc(step_ctr_d, 1))
             return false;
+        for (int i = 0; i < n; ++i)
+            ramp_on |= !(std::abs(E.ramp_duration[i]) < std::numeric_limits<double>::epsilon() * 100);
+        if (ramp_on) {
+            const size_t nv = (size_t)kRampRows * ns * n;
+            if (!alloc(ramp_tab_d, nv) || !alloc(ramp_base_d, 1)) return false;
+            hipError_t e = hipHostMalloc((void**)&ramp_tab_h, (nv + 1) * sizeof(double), hipHostMallocDefault);
+            if (e != hipSuccess) return fail("hipHostMalloc", e);
+        }
         return hipMemset(step_ctr_d, 0, sizeof(unsigned long long)) == hipSuccess ||
                fail("hipMemset", hipGetLastError());
     }
@@ -515,7 +535,7 @@ struct Engine final : EngineBase {
         }
         if (ens) {  // per-instance stepper advance, after the NaN keys of this stage are taken
             e = rkdg::launch_batch_advance(M.ens.n, ns, inst_t_d, inst_dt_d, inst_rd_d, inst_step_d, inst_stage_d, drk_d,
-                                           inst_tr_d, step_ctr_d, stream);
+                                           inst_tr_d, step_ctr_d, ramp_tab_d, ramp_base_d, kRampRows, stream);
             if (e != hipSuccess) return fail("advance launch", e), RKDG_ERR_HIP;
         }
         return RKDG_OK;
@@ -567,6 +587,43 @@ struct Engine final : EngineBase {
     rkdg_status inst_set(int i, const double* qh, const uint8_t* wh) override {
         return inst_copy(i, nullptr, qh, nullptr, wh);
     }
+    bool ramps() const override { return ramp_on; }
+    // The ramp of every instance at every stage of steps step..step+nsteps, the arithmetic of the
+    // host stepper (ESSPRKStepper::operator++ / GetRamp, explicit_ssp_rk_stepper.hpp:46-65: t +=
+    // dt per step, stage time t + dt drk[s], tanh(2 (t_s / 86400) / ramp_duration)), so every
+    // instance ramps bit for bit as its single-mesh run; the device stepper looks the values up.
+    // Waits for the stream (the staging copy is reused and the device time is read).
+    rkdg_status ramp_prepare(int nsteps) override {
+        if (!ramp_on) return RKDG_OK;
+        if (nsteps < 0 || nsteps >= kRampRows || M.stepper.stage != 0) return RKDG_ERR_ARG;
+        const int n = M.ens.n, ns = M.stepper.nstages;
+        std::vector<double> t(n);
+        std::vector<int> step(n);
+        hipError_t e = hipStreamSynchronize(stream);
+        if (e == hipSuccess) e = hipMemcpy(t.data(), inst_t_d, n * sizeof(double), hipMemcpyDeviceToHost);
+        if (e == hipSuccess) e = hipMemcpy(step.data(), inst_step_d, n * sizeof(int), hipMemcpyDeviceToHost);
+        if (e != hipSuccess) return fail("ramp table", e), RKDG_ERR_HIP;
+        const size_t nv = (size_t)kRampRows * ns * n;
+        for (int i = 0; i < n; ++i) {
+            const double dt = M.ens.dt[i], rd = M.ens.ramp_duration[i];
+            const bool no_ramp = std::abs(rd) < std::numeric_limits<double>::epsilon() * 100;
+            double ti = t[i];
+            for (int j = 0; j <= nsteps; ++j) {
+                for (int k = 0; k < ns; ++k) {
+                    const double ts = ti + dt * M.stepper.drk[k];
+                    ramp_tab_h[((size_t)j * ns + k) * n + i] = no_ramp ? 1.0 : std::tanh(2 * (ts / 86400) / rd);
+                }
+                ti += dt;
+            }
+        }
+        long long base = step[0];
+        std::memcpy(ramp_tab_h + nv, &base, sizeof base);
+        e = hipMemcpyAsync(ramp_tab_d, ramp_tab_h, (size_t)(nsteps + 1) * ns * n * sizeof(double), hipMemcpyHostToDevice,
+                           stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(ramp_base_d, ramp_tab_h + nv, sizeof base, hipMemcpyHostToDevice, stream);
+        if (e != hipSuccess) return fail("ramp table upload", e), RKDG_ERR_HIP;
+        return RKDG_OK;
+    }
     rkdg_status inst_time(int i, double* t, int* step, int* stage) override {
         if (M.ens.n == 0 || i < 0 || i >= M.ens.n) return RKDG_ERR_ARG;
         hipError_t e = hipStreamSynchronize(stream);
@@ -605,6 +662,8 @@ struct Engine final : EngineBase {
         if (qh) {
             std::vector<Real> tmp(qh, qh + S);
             hipError_t e = hipMemcpyAsync(q[M.stepper.stage], tmp.data(), S * sizeof(Real), hipMemcpyHostToDevice, stream);
+            // the new state replaces the one a recorded NaN was found in: clear the record
+            if (e == hipSuccess) e = hipMemcpyAsync(d_nan, &kNoNan, sizeof kNoNan, hipMemcpyHostToDevice, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
             if (e != hipSuccess) return fail("set_state", e), RKDG_ERR_HIP;
         }
@@ -853,7 +912,28 @@ static rkdg_status batch_capture(rkdg_ctx* ctx) {
     return RKDG_OK;
 }
 
+static rkdg_status batch_steps_chunk(rkdg_ctx* ctx, int nsteps);
+
+// Ensemble steps.  Instances that ramp their forcing take the ramp of each stage from a table the
+// host fills for up to kRampRows - 1 steps at a time (EngineBase::ramp_prepare).
 static rkdg_status batch_steps(rkdg_ctx* ctx, int nsteps) {
+    bool ramps = false;
+    for (auto& g : ctx->ge) ramps |= g->ramps();
+    if (!ramps) return batch_steps_chunk(ctx, nsteps);
+    while (nsteps > 0) {
+        const int n = std::min(nsteps, EngineBase::kRampRows - 1);
+        for (auto& g : ctx->ge) {
+            rkdg_status st = g->ramp_prepare(n);
+            if (st != RKDG_OK) return set_err(ctx, st, g->err);
+        }
+        rkdg_status st = batch_steps_chunk(ctx, n);
+        if (st != RKDG_OK) return st;
+        nsteps -= n;
+    }
+    return RKDG_OK;
+}
+
+static rkdg_status batch_steps_chunk(rkdg_ctx* ctx, int nsteps) {
     if (ctx->sched.graph) {
         if (!ctx->gexec) {
             rkdg_status st = batch_capture(ctx);

@@ -16,11 +16,14 @@ namespace rkdg {
 
 // ESSPRKStepper::operator++ (explicit_ssp_rk_stepper.hpp:46-65) for every ensemble instance, on
 // the device so that a captured step graph replays with the right stage times: stage <- (stage+1)
-// mod ns, on wrap t += dt and step += 1; stage time t + dt drk[stage]; ramp tanh(2 (t_s/86400)/
-// ramp_duration) unless ramp_duration is zero.  Built without FP contraction, like the host
-// stepper, so stage times match it bit for bit.
+// mod ns, on wrap t += dt and step += 1; stage time t + dt drk[stage].  Built without FP
+// contraction, like the host stepper, so stage times match it bit for bit.  The ramp
+// tanh(2 (t_s/86400)/ramp_duration) is not evaluated here: instances with a ramp duration read
+// it from ramp_tab ([row][stage][instance], row = step - *ramp_base), which the host fills with
+// its own tanh, so the ramp equals the single-mesh stepper's bit for bit.
 __global__ void batch_advance_kernel(int n, int ns, double* t, const double* dt, const double* rd, int* step, int* stage,
-                                     const double* drk, double* tr, unsigned long long* step_ctr) {
+                                     const double* drk, double* tr, unsigned long long* step_ctr,
+                                     const double* ramp_tab, const long long* ramp_base, int ramp_rows) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     int s = stage[i] + 1;
@@ -33,7 +36,10 @@ __global__ void batch_advance_kernel(int n, int ns, double* t, const double* dt,
     const double ts = t[i] + dt[i] * drk[s];
     tr[2 * i] = ts;
     const bool no_ramp = fabs(rd[i]) < 2.220446049250313e-16 * 100;
-    if (!no_ramp) tr[2 * i + 1] = tanh(2 * (ts / 86400) / rd[i]);
+    if (!no_ramp && ramp_tab) {
+        const long long row = (long long)step[i] - *ramp_base;
+        if (row >= 0 && row < ramp_rows) tr[2 * i + 1] = ramp_tab[((size_t)row * ns + s) * n + i];
+    }
     if (i == 0) *step_ctr = (unsigned long long)step[0];
 }
 
@@ -43,9 +49,10 @@ __global__ void copy_bytes_kernel(size_t n, const uint8_t* __restrict__ src, uin
 }
 
 hipError_t launch_batch_advance(int n, int ns, double* t, const double* dt, const double* rd, int* step, int* stage,
-                                const double* drk, double* tr, unsigned long long* step_ctr, hipStream_t s) {
+                                const double* drk, double* tr, unsigned long long* step_ctr, const double* ramp_tab,
+                                const long long* ramp_base, int ramp_rows, hipStream_t s) {
     hipLaunchKernelGGL(batch_advance_kernel, dim3((n + 63) / 64), dim3(64), 0, s, n, ns, t, dt, rd, step, stage, drk, tr,
-                       step_ctr);
+                       step_ctr, ramp_tab, ramp_base, ramp_rows);
     return hipGetLastError();
 }
 

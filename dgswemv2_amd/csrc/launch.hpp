@@ -49,7 +49,8 @@ hipError_t launch_mlp_pack(const float* theta, uint16_t* wpack, hipStream_t s);
 
 // Ensemble helpers (kernels_fp64.hip): per-instance stepper advance on the device, byte copy.
 hipError_t launch_batch_advance(int n, int ns, double* t, const double* dt, const double* rd, int* step, int* stage,
-                                const double* drk, double* tr, unsigned long long* step_ctr, hipStream_t s);
+                                const double* drk, double* tr, unsigned long long* step_ctr, const double* ramp_tab,
+                                const long long* ramp_base, int ramp_rows, hipStream_t s);
 hipError_t launch_copy_bytes(size_t n, const uint8_t* src, uint8_t* dst, hipStream_t s);
 
 }  // namespace rkdg

@@ -162,8 +162,11 @@ const char* rkdg_create_error(void);
 
 /* ---- state ------------------------------------------------------------------------ */
 
-/* Overwrites state[0].q (and the wet flags if wet != NULL); replaces direct writes into
- * SWE::Data::state (problem_data_structure/swe_data.hpp:13-70). */
+/* Overwrites the current stage's state (state[stage].q: state[0].q at a step boundary, the one
+ * rkdg_get_state reads) and the wet flags if wet != NULL; replaces direct writes into
+ * SWE::Data::state (problem_data_structure/swe_data.hpp:13-70).  Writing q also clears the NaN
+ * record, so a context that returned RKDG_ERR_NAN steps again from the new state (rkdg_nan_info
+ * reports none until a new NaN is found). */
 rkdg_status rkdg_set_state(rkdg_ctx* ctx, const double* q, const uint8_t* wet);
 /* Reads the current-stage state.  limited: [3][n_elements] per-variable slope-limiter
  * write-back flags of the last limiter pass (swe_CS_slope_limiter.hpp:180-187);
