@@ -10,6 +10,7 @@ graph replay all at once.
 import numpy as np
 import pytest
 
+from conftest import relerr
 from dgswemv2_amd import (RKDG_FP32_BULK, RKDG_FP64_PLAIN, Batch, RkdgError, Simulation, make_params,
                           workloads as W)
 
@@ -126,3 +127,69 @@ def test_batch_rejects_mixed_schemes():
     m2, p2 = W.dambreak(scale=0.03)
     with pytest.raises(RkdgError):
         Batch([m1, m2], [p1, p2], RKDG_FP64_PLAIN)
+
+
+def _area(mesh):
+    x, y = mesh.node_xyz[:, 0], mesh.node_xyz[:, 1]
+    e = mesh.elt_nodes
+    return 0.5 * np.abs((x[e[:, 1]] - x[e[:, 0]]) * (y[e[:, 2]] - y[e[:, 0]]) -
+                        (x[e[:, 2]] - x[e[:, 0]]) * (y[e[:, 1]] - y[e[:, 0]]))
+
+
+def test_ensemble_64_instances_full_size():
+    """The 64-instance ensemble at full size (triangle counts log-uniform over 20k-400k, 7.63 M in
+    all, p in {1, 2}, tide 0.1-1.5 m, SSP-RK3) in FP32 on the schedule the autotuner picks, 5 steps:
+    every instance equals its single-mesh run bit for bit and stays finite.  A land-only twin (the
+    tide boundary closed, started from the moving state) conserves each instance's mass.  The two
+    smallest instances (one of each order) run as an FP64-plain batch against the oracle per step."""
+    from dgswemv2_amd import Mesh, BC_LAND, BC_TIDE
+    from oracle.oracle_py import OracleSim
+    meshes, params, info = W.ensemble(n=64, scale=1.0)
+    banner("ensemble 64: triangles", sum(i["triangles"] for i in info))
+    b = Batch(meshes, params, RKDG_FP32_BULK)
+    best, ms = b.autotune(steps_per_trial=2)
+    banner("ensemble 64 autotuned", best, "ms/step", ms)
+    steps = 5
+    b.step(steps)
+    states = [b.state(i) for i in range(64)]
+    b.close()
+    for i in range(64):
+        assert np.all(np.isfinite(states[i]["q"])), i
+        s = Simulation(meshes[i], params[i], RKDG_FP32_BULK)
+        s.step(steps)
+        assert np.array_equal(states[i]["q"], s.state()["q"]), i
+        s.close()
+    banner("ensemble 64: every instance equals its single run")
+
+    # land-only twin from the moving state: mass (sum of mode 0 x area) kept to FP32 roundoff
+    land = [Mesh(m.node_xyz, m.elt_nodes, m.bound_edges, np.where(m.bound_types == BC_TIDE, BC_LAND, m.bound_types),
+                 node_manning_n=m.node_manning_n) for m in meshes]
+    t = Batch(land, params, RKDG_FP32_BULK)
+    t.set_schedule(**best)
+    for i in range(64):
+        t.set_state(i, states[i]["q"])
+    areas = [_area(m) for m in land]
+    m0 = [float(np.sum(states[i]["q"][0, 0].astype(np.float64) * areas[i])) for i in range(64)]
+    t.step(steps)
+    for i in range(64):
+        q = t.state(i)["q"]
+        m1 = float(np.sum(q[0, 0] * areas[i]))
+        scale = float(np.sum(np.abs(q[0, 0]) * areas[i])) + 1e-30
+        assert abs(m1 - m0[i]) <= 1e-6 * scale, (i, m0[i], m1)
+    t.close()
+    banner("ensemble 64: land-only twin conserves mass")
+
+    # the two smallest instances, one per order, in FP64 plain against the oracle per step
+    order = sorted(range(64), key=lambda i: info[i]["triangles"])
+    pick = [next(i for i in order if info[i]["p"] == 1), next(i for i in order if info[i]["p"] == 2)]
+    f = Batch([meshes[i] for i in pick], [params[i] for i in pick], RKDG_FP64_PLAIN)
+    f.set_schedule(**best)
+    orc = [OracleSim(meshes[i], params[i]) for i in pick]
+    for s in range(3):
+        f.step(1)
+        for k, o in enumerate(orc):
+            o.step(1)
+            r = max(relerr(f.state(k)["q"], o.state()["q"]))
+            assert r <= 1e-12, (pick[k], s, r)
+    banner("ensemble 64: smallest instances", [info[i]["triangles"] for i in pick], "match the oracle")
+    f.close()

@@ -273,3 +273,104 @@ def test_fp32_coarse_bowl_no_nan():
     s.step(30)
     assert np.all(np.isfinite(s.state()["q"]))
     s.close()
+
+
+# ---- whole runs of the first two workloads at their full sizes
+
+def test_manufactured_full_run_l2_residual():
+    """The manufactured workload as the reference runs it: 64x64 quads (8192 triangles), p = 1,
+    SSP-RK2, dt = 1e-3, 1000 steps.  The state is compared with the oracle every 100 steps, and
+    the L2 residual against the closed-form solution (Simulation::ComputeL2Residual,
+    simulation/serial/simulation.hpp:85-93; Dunavant rule 20 at the current stage time) is read
+    through the C-ABI (rkdg_l2_residual) and compared with the oracle's."""
+    mesh, prm = W.manufactured(scale=1.0)
+    banner("manufactured full run ne", mesh.n_elements, "1000 steps")
+    o = OracleSim(mesh, prm)
+    g = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    assert g.l2_residual() == pytest.approx(o.l2_residual(), rel=1e-13, abs=0)
+    worst = 0.0
+    for k in range(10):
+        o.step(100), g.step(100)
+        r = max(relerr(g.state()["q"], o.state()["q"]))
+        worst = max(worst, r)
+        assert r <= TOL64, (k, r)
+        lg, lo = g.l2_residual(), o.l2_residual()
+        assert lg == pytest.approx(lo, rel=1e-12, abs=0), (k, lg, lo)
+        assert g.time()[:2] == o.time()[:2]
+    banner("manufactured 1000 steps: worst rel L2", worst, "L2 residual", lg, "oracle", lo)
+    assert 0 < lg < 1e-3  # the discretisation error of a converged p = 1 run at this resolution
+
+
+def test_bowl_full_size_300_steps():
+    """The parabolic bowl at its full 128x128 size (32,768 triangles; wetting-drying and the
+    limiter), 300 steps, compared with the oracle every 50 steps, flags included."""
+    mesh, prm = W.bowl(scale=1.0)
+    banner("bowl full size ne", mesh.n_elements, "300 steps")
+    o = OracleSim(mesh, prm)
+    g = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    exact = True
+    for k in range(6):
+        o.step(50), g.step(50)
+        so, sg = o.state(), g.state()
+        r = max(relerr(sg["q"], so["q"]))
+        assert r <= TOL64, (k, r)
+        for f in ("wet", "limited", "troubled"):
+            assert np.array_equal(sg[f], so[f]), (k, f, int((sg[f] != so[f]).sum()))
+        exact &= bool(np.array_equal(sg["q"], so["q"]))
+    assert so["limited"].sum() > 0 and (so["wet"] == 0).sum() > 0  # the limiter and drying act
+    banner("bowl 300 steps bit-exact", exact)
+
+
+# ---- forcing ramp (ESSPRKStepper::GetRamp, explicit_ssp_rk_stepper.hpp:60-62)
+
+def _ramped_tidal(scale, p, seed, ramp_days):
+    mesh, prm = W.tidal(scale=scale, p=p, seed=seed)
+    prm.ramp_duration = ramp_days
+    return mesh, prm
+
+
+def test_tidal_ramp_single_vs_oracle():
+    """A nonzero ramp duration: the tide enters as tanh(2 (t_stage / 86400) / ramp_duration) times
+    its amplitude.  The duration is chosen so the ramp climbs from 0 to about 0.9 over the run."""
+    mesh, prm = W.tidal(scale=0.05)
+    steps = 12
+    prm.ramp_duration = 2.0 * steps * prm.dt / 86400 / 1.5
+    banner("tidal ramp ne", mesh.n_elements, "ramp days", prm.ramp_duration)
+    o = OracleSim(mesh, prm)
+    g = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    for i in range(steps):
+        o.step(1), g.step(1)
+        so, sg = o.state(), g.state()
+        r = max(relerr(sg["q"], so["q"]))
+        assert r <= TOL64, (i, r)
+    # the ramp matters: the same run without it differs
+    prm0 = W.tidal(scale=0.05)[1]
+    u = Simulation(mesh, prm0, RKDG_FP64_PLAIN)
+    u.step(steps)
+    assert max(relerr(u.state()["q"], so["q"])) > 1e-3
+    banner("tidal ramp bit-exact", bool(np.array_equal(sg["q"], so["q"])))
+
+
+@pytest.mark.parametrize("graph", [0, 1])
+def test_tidal_ramp_batch_vs_oracle(graph):
+    """Ensemble instances with different ramp durations (one without a ramp) and orders, stepped in
+    one batch (eager stages or graph replay): each follows the oracle's run of its mesh per step.
+    The ramp of each stage comes from the host's stepper arithmetic, so instances match their
+    single-mesh runs bit for bit."""
+    from dgswemv2_amd import Batch
+    cases = [_ramped_tidal(0.03, 1, 3, 0.0), _ramped_tidal(0.04, 2, 5, 2e-5), _ramped_tidal(0.03, 2, 7, 5e-5)]
+    b = Batch([c[0] for c in cases], [c[1] for c in cases], RKDG_FP64_PLAIN)
+    b.set_schedule(block=128, groups_per_xcd=0, graph=graph)
+    orc = [OracleSim(m, p) for m, p in cases]
+    single = [Simulation(m, p, RKDG_FP64_PLAIN) for m, p in cases]
+    for i in range(10):
+        b.step(1)
+        for k, (o, s) in enumerate(zip(orc, single)):
+            o.step(1), s.step(1)
+            got, ref = b.state(k)["q"], o.state()["q"]
+            r = max(relerr(got, ref))
+            assert r <= TOL64, (graph, i, k, r)
+            assert np.array_equal(got, s.state()["q"]), (graph, i, k)
+            assert b.time(k)[:2] == o.time()[:2]
+    banner("ramped ensemble graph", graph, "matches oracle and single runs")
+    b.close()
