@@ -444,6 +444,10 @@ class Simulation:
     def reset_timers(self):
         self._check(self.lib.rkdg_reset_timers(self.ctx))
 
+    def set_timing(self, per_kernel=True):
+        """Starts a timing window; per_kernel=False keeps only the events around each step call."""
+        self._check(self.lib.rkdg_set_timing(self.ctx, 1 if per_kernel else 0))
+
     def report(self):
         r = N.PerfReport()
         self._check(self.lib.rkdg_report(self.ctx, C.byref(r)))
@@ -641,6 +645,10 @@ class Batch:
 
     def reset_timers(self):
         self._check(self.lib.rkdg_reset_timers(self.ctx))
+
+    def set_timing(self, per_kernel=True):
+        """Starts a timing window; per_kernel=False keeps only the events around each step call."""
+        self._check(self.lib.rkdg_set_timing(self.ctx, 1 if per_kernel else 0))
 
     def report(self):
         r = N.PerfReport()

@@ -86,6 +86,7 @@ class QuadMeshSpec(C.Structure):
 
 
 MLP_IN, MLP_HIDDEN = 36, 64
+MLP_NPARAM = MLP_HIDDEN * MLP_IN + MLP_HIDDEN + 2 * (MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN) + MLP_HIDDEN + 1
 
 
 class MlpWeights(C.Structure):
@@ -131,6 +132,7 @@ SIGNATURES = {
     "rkdg_l2_residual": (C.c_int, [C.c_void_p, _dp]),
     "rkdg_report": (C.c_int, [C.c_void_p, C.POINTER(PerfReport)]),
     "rkdg_reset_timers": (C.c_int, [C.c_void_p]),
+    "rkdg_set_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "rkdg_master_tables": (C.c_int, [C.c_int32, _ip, _ip, _dp, _dp, _dp, _dp]),
     "rkdg_ssprk_coeffs": (C.c_int, [C.c_int32, C.c_int32, _dp, _dp, _dp]),
     "rkdg_mesh_generate": (C.c_int, [C.POINTER(QuadMeshSpec), C.POINTER(C.c_void_p), C.POINTER(MeshDesc)]),

@@ -1093,7 +1093,9 @@ rkdg_status rkdg_l2_residual(rkdg_ctx* ctx, double* l2) {
     return RKDG_OK;
 }
 
-rkdg_status rkdg_reset_timers(rkdg_ctx* ctx) {
+rkdg_status rkdg_reset_timers(rkdg_ctx* ctx) { return rkdg_set_timing(ctx, 1); }
+
+rkdg_status rkdg_set_timing(rkdg_ctx* ctx, int32_t per_kernel) {
     if (!ctx) return RKDG_ERR_ARG;
     hipStreamSynchronize(ctx->stream);
     rkdg_perf_report scratch{};
@@ -1104,7 +1106,7 @@ rkdg_status rkdg_reset_timers(rkdg_ctx* ctx) {
         engines.push_back(ctx->eng.get());
     for (EngineBase* g : engines) {
         g->collect(&scratch);
-        g->timing = true;
+        g->timing = per_kernel != 0;
         g->stages_timed = 0;
     }
     ctx->acc = rkdg_perf_report{};

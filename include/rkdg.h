@@ -157,7 +157,8 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
                         rkdg_ctx** out);
 void rkdg_destroy(rkdg_ctx* ctx);
 const char* rkdg_last_error(const rkdg_ctx* ctx);
-/* Message of the last rkdg_create failure (no context exists then). */
+/* Message of the last failure of a call without a context: rkdg_create, the mesh readers, writers
+ * and partitioner. */
 const char* rkdg_create_error(void);
 
 /* ---- state ------------------------------------------------------------------------ */
@@ -202,6 +203,11 @@ rkdg_status rkdg_l2_residual(rkdg_ctx* ctx, double* l2);
 /* Per-kernel HIP-event times and the algorithmic byte count of one stage launch. */
 rkdg_status rkdg_report(rkdg_ctx* ctx, rkdg_perf_report* report);
 rkdg_status rkdg_reset_timers(rkdg_ctx* ctx);
+/* Starts a timing window as rkdg_reset_timers does; with per_kernel = 0 no event is recorded
+ * between kernel launches (the launches of a step stay back to back on the stream) and only the
+ * HIP events around each rkdg_step call are kept (rkdg_perf_report.total_ms); rkdg_reset_timers
+ * is rkdg_set_timing(ctx, 1). */
+rkdg_status rkdg_set_timing(rkdg_ctx* ctx, int32_t per_kernel);
 
 /* ---- host-side helpers (no device work) ------------------------------------------- */
 
@@ -452,6 +458,7 @@ rkdg_status rkdg_mlp_forward(rkdg_mlp_trainer* t, const float* x, int32_t n, flo
 rkdg_status rkdg_mlp_gradient(rkdg_mlp_trainer* t, const float* x, const float* y, int32_t n, float pos_weight,
                               float* loss, rkdg_mlp_weights* grad);
 const char* rkdg_mlp_trainer_error(const rkdg_mlp_trainer* t);
+
 void rkdg_mlp_trainer_destroy(rkdg_mlp_trainer* t);
 
 #ifdef __cplusplus

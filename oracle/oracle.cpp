@@ -600,6 +600,7 @@ struct Sim {
     double h_o = 0.1, h_o_threshold = 1.0e-6, M = 1.0e-8, nu = 1.5;
     int nthreads = 1;
     bool keep_prelimit = false;       // keep each stage's state as it was before the limiter
+    bool round_state = false;         // round the state to FP32 at the end of every stage (FP32-storage floor)
     std::vector<double> prelimit;     // [3][ndof][ne]
     std::vector<std::vector<unsigned>> intface_colours;  // for the OpenMP path
     int nan_elt = -1, nan_step = -1, nan_stage = -1;
@@ -1625,6 +1626,9 @@ static int stage(Sim& S) {
                     S.prelimit[(k * nd + d) * ne + e] = S.elts[e].q[S.stepper.stage](k, (int)d);
     }
     if (S.prm.slope_limiting) slope_limiter(S);
+    if (S.round_state)
+        for (auto& E : S.elts)
+            for (double& v : E.q[S.stepper.stage].a) v = (double)(float)v;
     for (auto& E : S.elts)
         if (scrutinize(S, E)) {
             S.nan_elt = (int)E.id, S.nan_step = (int)old_step, S.nan_stage = (int)old_stage;
@@ -1673,6 +1677,9 @@ const char* orc_error() { return orc::g_err; }
 void orc_destroy(void* h) { delete (orc::Sim*)h; }
 void orc_set_threads(void* h, int n) { ((orc::Sim*)h)->nthreads = n; }
 void orc_keep_prelimit(void* h, int on) { ((orc::Sim*)h)->keep_prelimit = on != 0; }
+// Rounds the state to FP32 at the end of every stage (everything else stays FP64): the error any
+// form that stores its state in FP32 carries at least.
+void orc_round_state(void* h, int on) { ((orc::Sim*)h)->round_state = on != 0; }
 // Neighbour element across each local face ([3][ne], -1 on a boundary) and the bathymetry at
 // each barycentre ([ne], only with the limiter): the inputs of the learned indicator's features.
 void orc_limiter_geometry(void* h, int* neighbor, double* bath_bc) {

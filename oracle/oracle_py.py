@@ -28,6 +28,7 @@ def load():
             "orc_destroy": (None, [C.c_void_p]),
             "orc_set_threads": (None, [C.c_void_p, C.c_int]),
             "orc_keep_prelimit": (None, [C.c_void_p, C.c_int]),
+            "orc_round_state": (None, [C.c_void_p, C.c_int]),
             "orc_get_prelimit": (C.c_int, [C.c_void_p, dp]),
             "orc_limiter_geometry": (None, [C.c_void_p, ip, dp]),
             "orc_max_threads": (C.c_int, []),
@@ -106,6 +107,10 @@ class OracleSim:
         q = np.ascontiguousarray(q, np.float64).reshape(-1)
         w = None if wet is None else np.ascontiguousarray(wet, np.uint8)
         self.lib.orc_set_state(self.h, N.dptr(q), N.uptr(w))
+
+    def round_state(self, on=True):
+        """Round the state to FP32 at the end of every stage (the FP32-storage error floor)."""
+        self.lib.orc_round_state(self.h, 1 if on else 0)
 
     def keep_prelimit(self, on=True):
         """Keep each stage's state as it was after wetting-drying, before the limiter."""
