@@ -775,7 +775,9 @@ static double stage_bytes(const rkdg::Model& M, rkdg_precision precision, double
         }
     }
     reads_q /= ns;
-    double per_elt = (reads_q + 1.0) * S + M.ndof * b + 5 * b + 28.0;
+    // modal bathymetry: every mode in the plain form, the three linear ones in the fast forms
+    const int nbath = precision == RKDG_FP64_PLAIN ? M.ndof : std::min(M.ndof, 3);
+    double per_elt = (reads_q + 1.0) * S + nbath * b + 5 * b + 28.0;
     if (M.prm.wetting_drying) per_elt += 2.0 + 24.0;
     return per_elt * ne_real + 3.0 * b * M.nedge;
 }
