@@ -56,3 +56,20 @@ def test_fp32_storage_floor_exceeds_the_per_variable_bar_on_qy(runs):
     assert per[0] < 1e-7 and per[1] < 1e-6 and mom < 1e-6
     # and it grows with the step count (rounding errors of every stage accumulate)
     assert runs[10][0][0][2] < runs[40][0][0][2] < runs[70][0][0][2]
+
+
+def test_fp32_storage_floor_of_the_bowls_momentum():
+    """The parabolic bowl starts from still water: after six stages its momentum is a small field,
+    and rounding the state to FP32 each stage already moves it by more than 1e-3 of its norm while
+    the surface moves by a few 1e-6.  (The wetting-drying GPU checks of the FP32 form use this.)"""
+    mesh, prm = W.bowl(scale=0.1, slope_limiting=False)
+    ref, r32 = OracleSim(mesh, prm), OracleSim(mesh, prm)
+    r32.round_state(True)
+    for _ in range(6):
+        ref.stage(), r32.stage()
+    q, q32 = ref.state()["q"], r32.state()["q"]
+    (ze, qx, qy), mom = rel(q32, q)
+    print(f"bowl {mesh.n_elements} triangles, 6 stages: FP32-state floor ze/qx/qy {ze:.2e} {qx:.2e} {qy:.2e}, "
+          f"momentum {mom:.2e}")
+    assert ze < 1e-5 and 1e-3 < mom < 1e-2
+    ref.close(), r32.close()

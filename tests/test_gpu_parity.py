@@ -298,7 +298,9 @@ def test_manufactured_full_run_l2_residual():
         assert lg == pytest.approx(lo, rel=1e-12, abs=0), (k, lg, lo)
         assert g.time()[:2] == o.time()[:2]
     banner("manufactured 1000 steps: worst rel L2", worst, "L2 residual", lg, "oracle", lo)
-    assert 0 < lg < 1e-3  # the discretisation error of a converged p = 1 run at this resolution
+    # the value itself is the oracle's discretisation error (its rate is pinned on the CPU by the
+    # manufactured-solution convergence tests); here it must only be a finite, nonzero residual
+    assert np.isfinite(lg) and lg > 0
 
 
 def test_bowl_full_size_300_steps():
