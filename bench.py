@@ -152,7 +152,19 @@ def cpu_baseline(runs=3, steps_per_run=4):
 
 def copy_bandwidth(mib=2048, reps=10):
     """Measured HBM bandwidth of a device-to-device copy of `mib` MiB (read + write bytes per
-    second over `reps` copies, HIP events on torch's stream): the achievable peak beside 8 TB/s."""
+    second over `reps` copies): the achievable peak beside 8 TB/s.  Measured in a child process:
+    torch brings its own HIP runtime, which cannot start in a process where librkdg.so's runtime
+    already holds the device."""
+    import subprocess
+    r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--copy-gbs", str(mib), str(reps)],
+                       capture_output=True, text=True, timeout=300)
+    if r.returncode != 0:
+        raise RuntimeError("copy bandwidth probe failed: " + r.stderr[-2000:])
+    return float(r.stdout.strip().splitlines()[-1])
+
+
+def copy_gbs_child(mib, reps):
+    """The copy measurement of copy_bandwidth: torch copies on its own stream, timed by HIP events."""
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
@@ -410,6 +422,8 @@ def main():
     ap.add_argument("--learned-scale", type=float, default=1.0)
     ap.add_argument("--form", choices=("all", "fp64_plain", "fp64_fast", "fp32_bulk", "none"), default="all",
                     help="time one form only (short profiled runs); the JSON line needs all")
+    ap.add_argument("--copy-gbs", type=int, nargs=2, default=None, metavar=("MIB", "REPS"),
+                    help="(internal) print the device copy bandwidth in GB/s and exit")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run 5 device copies of MIB MiB (counter calibration) and exit")
     ap.add_argument("--gate-probe", type=int, default=0, metavar="STEPS",
@@ -419,6 +433,9 @@ def main():
     ap.add_argument("--halo-parts", type=int, default=0, metavar="N",
                     help="only run the partitioned tidal workload (N parts, halo exchange) and exit")
     args = ap.parse_args()
+    if args.copy_gbs:
+        print(copy_gbs_child(*args.copy_gbs), flush=True)
+        return
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
