@@ -173,6 +173,7 @@ SIGNATURES = {
     "rkdg_mlp_train": (C.c_int, [C.c_void_p, C.POINTER(TrainParams), _fp]),
     "rkdg_mlp_get_weights": (C.c_int, [C.c_void_p, C.POINTER(MlpWeights)]),
     "rkdg_mlp_forward": (C.c_int, [C.c_void_p, _fp, C.c_int32, _fp]),
+    "rkdg_mlp_forward_noalloc": (C.c_int, [C.POINTER(MlpWeights), _fp, C.c_int32, _fp]),
     "rkdg_mlp_gradient": (C.c_int, [C.c_void_p, _fp, _fp, C.c_int32, C.c_float, _fp, C.POINTER(MlpWeights)]),
     "rkdg_mlp_trainer_error": (C.c_char_p, [C.c_void_p]),
     "rkdg_mlp_trainer_destroy": (None, [C.c_void_p]),

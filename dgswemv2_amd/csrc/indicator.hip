@@ -7,6 +7,10 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
+#include <mutex>
+
+#include "../../include/rkdg.h"
 
 #include "engine.hpp"
 #include "indicator.hpp"
@@ -332,7 +336,7 @@ __global__ void __launch_bounds__(256) mlp_fwdbwd_kernel(TrainArgs T) {
     if (tile * 16 >= T.Bpad) return;  // whole wave
     const int p = tile * 16 + n;
     const bool valid = p < T.B;
-    const long long sidx = valid ? (long long)T.idx[p] : 0;
+    const long long sidx = valid ? (T.idx ? (long long)T.idx[p] : (long long)p) : 0;
     float x[9];
 #pragma unroll
     for (int s = 0; s < 9; ++s) x[s] = valid ? T.X[(long long)(9 * g + s) * T.ldx + sidx] : 0.0f;
@@ -558,6 +562,62 @@ hipError_t launch_mlp_update(int nslices, const float* part, const float* loss_p
 hipError_t launch_mlp_pack(const float* theta, uint16_t* wpack, hipStream_t s) {
     hipLaunchKernelGGL(mlp_pack_kernel, dim3((mlp::NPACKED * mlp::PACK + 255) / 256), dim3(256), 0, s, theta, wpack);
     return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------------------------------------------
+// Allocation-free inference (rkdg_mlp_forward_noalloc).  The staging that rkdg_mlp_forward keeps in
+// a trainer's device buffers (master parameters, packed bf16 weights, features, logits) lives here
+// in one file-scope device array of fixed size.  Each GPU holds its own copy of the array, at its
+// own address: the address is looked up once per device with hipGetSymbolAddress and kept, and only
+// that address is handed to kernels and copies (in host code the array's name is a stand-in, not a
+// device address).  Calls in one process share the array and are serialised by a lock; every
+// region a call reads is written by that call first, so nothing needs clearing.
+namespace {
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr int kFwdMax = RKDG_MLP_FORWARD_NOALLOC_MAX;
+static_assert(kFwdMax % 32 == 0, "the forward kernel writes logits up to the 32-padded sample count");
+constexpr size_t kOffTheta = 0;
+constexpr size_t kOffWpack = al256(kOffTheta + (size_t)mlp::NPARAM * sizeof(float));
+constexpr size_t kOffX = al256(kOffWpack + (size_t)mlp::NPACKED * mlp::PACK * sizeof(uint16_t));
+constexpr size_t kOffLogit = al256(kOffX + (size_t)mlp::NIN * kFwdMax * sizeof(float));
+constexpr size_t kScratchBytes = al256(kOffLogit + (size_t)kFwdMax * sizeof(float));
+constexpr int kMaxDevices = 64;
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+hipError_t mlp_forward_noalloc(const float* theta_h, const float* x, int n, float* logit) {
+    if (n < 1 || n > kFwdMax) return hipErrorInvalidValue;
+    static std::mutex mu;
+    static void* base[kMaxDevices] = {};
+    std::lock_guard<std::mutex> lock(mu);
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    if (!base[dev]) {  // once per device, with that device current
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        base[dev] = ptr;
+    }
+    unsigned char* s = static_cast<unsigned char*>(base[dev]);
+    float* theta = reinterpret_cast<float*>(s + kOffTheta);
+    uint16_t* wpack = reinterpret_cast<uint16_t*>(s + kOffWpack);
+    float* X = reinterpret_cast<float*>(s + kOffX);
+    float* lg = reinterpret_cast<float*>(s + kOffLogit);
+    e = hipMemcpy(theta, theta_h, (size_t)mlp::NPARAM * sizeof(float), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_mlp_pack(theta, wpack, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(X, x, (size_t)mlp::NIN * n * sizeof(float), hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+    TrainArgs T;
+    memset(&T, 0, sizeof T);
+    T.B = n, T.Bpad = (n + 31) / 32 * 32, T.X = X, T.ldx = n, T.idx = nullptr, T.pos_weight = 1.0f;
+    T.wpack = wpack, T.theta = theta, T.logit = lg;
+    e = launch_mlp_fwdbwd(T, false, nullptr);
+    if (e == hipSuccess) e = hipMemcpy(logit, lg, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+    return e;
 }
 
 }  // namespace rkdg

@@ -248,6 +248,13 @@ rkdg_status rkdg_mlp_gradient(rkdg_mlp_trainer* t, const float* x, const float* 
     return RKDG_OK;
 }
 
+rkdg_status rkdg_mlp_forward_noalloc(const rkdg_mlp_weights* w, const float* x, int32_t n, float* logit) {
+    if (!w || !x || !logit || n < 1 || n > RKDG_MLP_FORWARD_NOALLOC_MAX) return RKDG_ERR_ARG;
+    if (!rkdg_mlp_trainer::finite(x, (size_t)rkdg::mlp::NIN * n)) return RKDG_ERR_ARG;
+    return rkdg::mlp_forward_noalloc(reinterpret_cast<const float*>(w), x, n, logit) == hipSuccess ? RKDG_OK
+                                                                                                  : RKDG_ERR_HIP;
+}
+
 const char* rkdg_mlp_trainer_error(const rkdg_mlp_trainer* t) { return t ? t->err.c_str() : "null trainer"; }
 
 void rkdg_mlp_trainer_destroy(rkdg_mlp_trainer* t) { delete t; }

@@ -457,6 +457,17 @@ rkdg_status rkdg_mlp_forward(rkdg_mlp_trainer* t, const float* x, int32_t n, flo
 /* Loss and gradient (weights layout; threshold 0) of one batch x [36][n], y [n], n <= capacity. */
 rkdg_status rkdg_mlp_gradient(rkdg_mlp_trainer* t, const float* x, const float* y, int32_t n, float pos_weight,
                               float* loss, rkdg_mlp_weights* grad);
+/* Allocation-free variant of rkdg_mlp_forward: logits of x [36][n] (host) under the weights w
+ * (host), with no trainer.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It calls no hipMalloc / hipFree: its staging (parameters, packed weights, features,
+ * logits; about 2.5 MB) is a fixed device array that each GPU holds for the life of the process.
+ * n <= RKDG_MLP_FORWARD_NOALLOC_MAX, else RKDG_ERR_ARG.  Calls in one process share that array
+ * and are serialised by a lock (concurrent callers wait); runs on the current device's null
+ * stream and returns when the logits are on the host.  Same kernel and results as
+ * rkdg_mlp_forward on the same weights. */
+#define RKDG_MLP_FORWARD_NOALLOC_MAX 16384
+rkdg_status rkdg_mlp_forward_noalloc(const rkdg_mlp_weights* w, const float* x, int32_t n, float* logit);
 const char* rkdg_mlp_trainer_error(const rkdg_mlp_trainer* t);
 
 void rkdg_mlp_trainer_destroy(rkdg_mlp_trainer* t);

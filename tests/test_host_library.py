@@ -58,3 +58,17 @@ def test_mesh_generator_patterns():
         assert np.all(area > 0)  # counter-clockwise, as quad_mesh_generator writes them
         assert np.isclose(area.sum() / 2, 6.0)
         assert len(m.bound_types) == 2 * (3 + 2)
+
+
+def test_noalloc_forward_rejects_bad_arguments_before_any_device_work():
+    lib = N.load_library()
+    w = N.MlpWeights()
+    X = np.zeros((N.MLP_IN, 4), np.float32)
+    out = np.zeros(4, np.float32)
+    limit = 16384  # RKDG_MLP_FORWARD_NOALLOC_MAX
+    assert "#define RKDG_MLP_FORWARD_NOALLOC_MAX 16384" in HEADER.read_text()
+    assert lib.rkdg_mlp_forward_noalloc(None, N.fptr(X), 4, N.fptr(out)) == N.RKDG_ERR_ARG
+    assert lib.rkdg_mlp_forward_noalloc(C.byref(w), N.fptr(X), 0, N.fptr(out)) == N.RKDG_ERR_ARG
+    assert lib.rkdg_mlp_forward_noalloc(C.byref(w), N.fptr(X), limit + 1, N.fptr(out)) == N.RKDG_ERR_ARG
+    X[3, 1] = np.nan
+    assert lib.rkdg_mlp_forward_noalloc(C.byref(w), N.fptr(X), 4, N.fptr(out)) == N.RKDG_ERR_ARG
