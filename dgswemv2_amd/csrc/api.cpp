@@ -66,6 +66,8 @@ struct EngineBase {
     virtual rkdg_status halo_define(int slot, const int32_t* idx, int n) = 0;
     virtual rkdg_status halo_buffer(int slot, void** buf, int64_t* bytes) = 0;
     virtual rkdg_status halo_move(int slot, bool pack, const void* src, int64_t src_bytes) = 0;
+    // work-groups of this engine's persistent stage kernel resident on one CU at this block size
+    virtual int persistent_resident(int block) const = 0;
     int block = 128;            // work-items per block (measured best for the stage kernels on 1M elements)
     int persistent_grid = 0;    // > 0: persistent stage kernel on this many work-groups (ensembles)
     bool capturing = false;     // inside a graph capture: no host-side event records
@@ -129,6 +131,14 @@ struct Engine final : EngineBase {
                                                    persistent_grid);
         else
             return hipErrorInvalidValue;  // FP32 exists only in the fast form
+    }
+    int persistent_resident(int b) const override {
+        const bool wd = M.prm.wetting_drying != 0;
+        if constexpr (sizeof(Real) == 8)
+            return fast ? rkdg::stage_persistent_resident<Real, true>(M.p, wd, b)
+                        : rkdg::stage_persistent_resident<Real, false>(M.p, wd, b);
+        else
+            return rkdg::stage_persistent_resident<Real, true>(M.p, wd, b);
     }
     hipError_t launch_fast(const rkdg::StageArgs<Real>& A) {
         return rkdg::launch_stage<Real, true>(M.p, M.prm.wetting_drying != 0, M.closed_forms, A, stream, block,
@@ -1305,7 +1315,7 @@ rkdg_status rkdg_batch_nan_info(rkdg_ctx* ctx, int32_t* instance, int32_t* eleme
 }
 
 static bool schedule_valid(const rkdg_schedule& s) {
-    return (s.block == 64 || s.block == 128 || s.block == 256) && s.groups_per_xcd >= 0 &&
+    return (s.block == 64 || s.block == 128 || s.block == 256) && s.groups_per_xcd >= -1 &&
            s.groups_per_xcd <= 4096 && (s.graph == 0 || s.graph == 1);
 }
 
@@ -1314,11 +1324,25 @@ rkdg_status rkdg_batch_set_schedule(rkdg_ctx* ctx, const rkdg_schedule* sc) {
     if (!ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "not a batch context");
     if (!schedule_valid(*sc)) return set_err(ctx, RKDG_ERR_ARG, "invalid schedule");
     for (auto& g : ctx->gm)
-        if (sc->groups_per_xcd > 0 && g->closed_forms)
+        if (sc->groups_per_xcd != 0 && g->closed_forms)
             return set_err(ctx, RKDG_ERR_ARG, "the persistent kernel is not built for closed-form problems");
+    // -1: each order's persistent kernel on as many work-groups as are resident at once on every CU
+    std::vector<int> grid(ctx->ge.size());
+    for (size_t k = 0; k < ctx->ge.size(); ++k) {
+        if (sc->groups_per_xcd >= 0) {
+            grid[k] = 8 * sc->groups_per_xcd;
+            continue;
+        }
+        int dev = 0, cu = 0;
+        hipGetDevice(&dev);
+        hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev);
+        const int r = ctx->ge[k]->persistent_resident(sc->block);
+        if (r < 1 || cu < 8) return set_err(ctx, RKDG_ERR_HIP, "occupancy of the persistent stage kernel unknown");
+        grid[k] = 8 * (cu / 8) * r;
+    }
     hipStreamSynchronize(ctx->stream);
     ctx->sched = *sc;
-    for (auto& g : ctx->ge) g->block = sc->block, g->persistent_grid = 8 * sc->groups_per_xcd;
+    for (size_t k = 0; k < ctx->ge.size(); ++k) ctx->ge[k]->block = sc->block, ctx->ge[k]->persistent_grid = grid[k];
     ctx->drop_graph();
     return RKDG_OK;
 }
@@ -1338,15 +1362,13 @@ rkdg_status rkdg_batch_autotune(rkdg_ctx* ctx, int32_t steps_per_trial, rkdg_sch
     }
     bool cf = false;
     for (auto& g : ctx->gm) cf |= g->closed_forms;
-    int cu = 0;
-    hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, 0);
-    const int cu_per_xcd = std::max(1, cu / 8);
+    // one work-group per tile, or the persistent kernel on the resident work-groups (-1)
     std::vector<rkdg_schedule> cand;
     for (int b : {64, 128, 256})
-        for (int mult : {0, 1, 2, 4})
+        for (int pers : {0, -1})
             for (int gr : {0, 1}) {
-                if (mult && cf) continue;
-                cand.push_back(rkdg_schedule{b, mult * cu_per_xcd, gr});
+                if (pers && cf) continue;
+                cand.push_back(rkdg_schedule{b, pers, gr});
             }
     const bool was_timing = ctx->timing;
     rkdg_schedule win = ctx->sched;

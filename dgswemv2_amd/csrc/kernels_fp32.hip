@@ -3,6 +3,7 @@
 #include "kernels.inc"
 
 namespace rkdg {
+template int stage_persistent_resident<float, true>(int, bool, int);
 template hipError_t launch_stage<float, true>(int, bool, bool, const StageArgs<float>&, hipStream_t, int, int);
 template hipError_t launch_limiter<float>(int, const LimiterArgs<float>&, hipStream_t, int);
 template hipError_t launch_limiter_prepare<float>(int, const LimiterArgs<float>&, hipStream_t, int);

@@ -3,6 +3,7 @@
 #include "kernels.inc"
 
 namespace rkdg {
+template int stage_persistent_resident<double, false>(int, bool, int);
 template hipError_t launch_stage<double, false>(int, bool, bool, const StageArgs<double>&, hipStream_t, int, int);
 template hipError_t launch_limiter<double>(int, const LimiterArgs<double>&, hipStream_t, int);
 template hipError_t launch_limiter_prepare<double>(int, const LimiterArgs<double>&, hipStream_t, int);

@@ -3,5 +3,6 @@
 #include "kernels.inc"
 
 namespace rkdg {
+template int stage_persistent_resident<double, true>(int, bool, int);
 template hipError_t launch_stage<double, true>(int, bool, bool, const StageArgs<double>&, hipStream_t, int, int);
 }  // namespace rkdg

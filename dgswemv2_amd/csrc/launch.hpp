@@ -15,6 +15,10 @@ namespace rkdg {
 template <typename Real, bool FAST>
 hipError_t launch_stage(int p, bool wd, bool cf, const StageArgs<Real>& A, hipStream_t s, int block,
                         int persistent_grid = 0);
+// Work-groups of the ensemble's persistent stage kernel (order p, wetting-drying wd) resident on
+// one CU at once at this block size; 0 if the query fails.
+template <typename Real, bool FAST>
+int stage_persistent_resident(int p, bool wd, int block);
 
 template <typename Real>
 hipError_t launch_limiter(int p, const LimiterArgs<Real>& L, hipStream_t s, int block);
@@ -46,6 +50,9 @@ hipError_t launch_mlp_update(int nslices, const float* part, const float* loss_p
                              float* m1, float* m2, float lr, float b1, float b2, float eps, float c1, float c2,
                              bool update, float* loss_out, hipStream_t s);
 hipError_t launch_mlp_pack(const float* theta, uint16_t* wpack, hipStream_t s);
+// Logits of x [36][n] (host) under the flat parameters theta (host), staged through the library's
+// fixed device array (no allocation); n <= RKDG_MLP_FORWARD_NOALLOC_MAX.
+hipError_t mlp_forward_noalloc(const float* theta, const float* x, int n, float* logit);
 
 // Ensemble helpers (kernels_fp64.hip): per-instance stepper advance on the device, byte copy.
 hipError_t launch_batch_advance(int n, int ns, double* t, const double* dt, const double* rd, int* step, int* stage,

@@ -258,7 +258,9 @@ rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* spec, rkdg_mesh** out,
 typedef struct rkdg_schedule {
     int32_t block;          /* work-items per block: 64, 128 or 256 */
     int32_t groups_per_xcd; /* 0: one work-group per element tile; > 0: persistent stage kernel with
-                               this many work-groups on each of the 8 XCDs */
+                               this many work-groups on each of the 8 XCDs; -1: persistent stage
+                               kernel on as many work-groups as are resident at once on every CU
+                               (from the kernel's occupancy at this block size, per order) */
     int32_t graph;          /* 1: each step replays one captured hipGraph (all stages, all orders) */
 } rkdg_schedule;
 
@@ -273,9 +275,10 @@ rkdg_status rkdg_batch_get_time(rkdg_ctx* ctx, int32_t instance, double* t, int3
 rkdg_status rkdg_batch_nan_info(rkdg_ctx* ctx, int32_t* instance, int32_t* element, int32_t* step, int32_t* stage);
 rkdg_status rkdg_batch_set_schedule(rkdg_ctx* ctx, const rkdg_schedule* schedule);
 rkdg_status rkdg_batch_get_schedule(const rkdg_ctx* ctx, rkdg_schedule* schedule);
-/* Times steps_per_trial steps under each candidate schedule (block 64/128/256 x one-group-per-
- * tile or 1, 2, 4 persistent work-groups per CU x graph off/on) from the same state, keeps the
- * fastest, and restores the state; best_ms_per_step may be NULL. */
+/* Times steps_per_trial steps under each candidate schedule (block 64/128/256 x one work-group
+ * per tile or the persistent kernel on the resident work-groups (groups_per_xcd = -1) x graph
+ * off/on) from the same state, keeps the fastest, and restores the state; best_ms_per_step may
+ * be NULL. */
 rkdg_status rkdg_batch_autotune(rkdg_ctx* ctx, int32_t steps_per_trial, rkdg_schedule* best,
                                 double* best_ms_per_step);
 

@@ -17,6 +17,7 @@ from dgswemv2_amd import (RKDG_FP32_BULK, RKDG_FP64_PLAIN, Batch, RkdgError, Sim
 pytestmark = pytest.mark.gpu
 
 SCHEDULES = [dict(block=128, groups_per_xcd=0, graph=0), dict(block=64, groups_per_xcd=32, graph=0),
+             dict(block=64, groups_per_xcd=-1, graph=0), dict(block=256, groups_per_xcd=-1, graph=1),
              dict(block=256, groups_per_xcd=0, graph=1), dict(block=128, groups_per_xcd=64, graph=1)]
 
 
