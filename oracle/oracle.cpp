@@ -283,9 +283,29 @@ static Master make_master(unsigned p) {
     return m;
 }
 
+// Planted mistakes, for the tests that pin the limiting step (tests/test_oracle_limiter.py) and the
+// edge geometry (tests/test_oracle_geometry.py): each check there has a twin run with one of these
+// switched on that must fail.  0: the reference's arithmetic.  Set only by those tests (orc_set_plant); never on in any other use of the oracle.
+enum {
+    PLANT_NONE = 0,
+    PLANT_INVR_SIGN = 1,    // invR(0,1) with the wrong sign
+    PLANT_ALPHA_SWAP = 2,   // alpha columns exchanged in w_delta
+    PLANT_T_SIGN = 3,       // vertex rebuild with T = +1 on the diagonal, -1 off it
+    PLANT_THETA_SWAP = 4,   // theta+ and theta- exchanged
+    PLANT_NU_ONE = 5,       // minmod against |w_delta| instead of |nu w_delta|
+    PLANT_R_TRANSPOSE = 6,  // R transposed in the characteristic back-transform
+    PLANT_NO_THETA = 7,     // positive / negative balance skipped
+    PLANT_TVB_OFF = 8,      // |w_tilda| <= M r^2 branch skipped
+    PLANT_NORMAL_SIGN = 9,  // edge normals of elements' own sides flipped
+    PLANT_SURFJ_FULL = 10,  // edge surface Jacobian L instead of L / 2
+    PLANT_EDGE_SHIFT = 11   // edge k's Gauss points placed on master edge (k + 1) % 3
+};
+static int g_plant = PLANT_NONE;
+
 // Edge k of the master triangle for GL points z (BoundaryToMasterCoordinates :89-122).
 static std::vector<std::array<double, 2>> boundary_to_master(unsigned bound_id, const std::vector<double>& z) {
     std::vector<std::array<double, 2>> out(z.size());
+    if (g_plant == PLANT_EDGE_SHIFT) bound_id = (bound_id + 1) % 3;
     for (size_t g = 0; g < z.size(); ++g) {
         if (bound_id == 0)
             out[g] = {-z[g], z[g]};
@@ -615,6 +635,7 @@ static void make_side(const Sim& S, EdgeSide& s, const Element& e, unsigned boun
     auto zm = boundary_to_master(bound, rule.z);
     s.gp_global = e.shape.local_to_global(zm);
     dubiner(S.master.p, zm, &s.phi, nullptr, nullptr);
+    if (g_plant == PLANT_SURFJ_FULL) sJ = 2.0 * sJ;
     s.int_fact.resize(ngp);
     for (unsigned g = 0; g < ngp; ++g) s.int_fact[g] = rule.w[g] * sJ;
     s.int_phi_fact = Mat(ngp, S.master.ndof);
@@ -623,7 +644,7 @@ static void make_side(const Sim& S, EdgeSide& s, const Element& e, unsigned boun
     s.normal.resize(ngp);
     for (unsigned g = 0; g < ngp; ++g) {
         if (!reversed_normal)
-            s.normal[g] = normal;
+            s.normal[g] = g_plant == PLANT_NORMAL_SIGN ? std::array<double, 2>{-normal[0], -normal[1]} : normal;
         else
             s.normal[ngp - g - 1] = {-normal[0], -normal[1]};
     }
@@ -1345,21 +1366,6 @@ static void wetting_drying_kernel(Sim& S, Element& E) {  // swe_post_wet_dry.hpp
 
 // ---- slope limiter (swe_CS_slope_limiter.hpp, swe_trouble_check.hpp, swe_CS_sl_serial.hpp)
 
-// Planted mistakes, for the tests that pin the limiting step (tests/test_oracle_limiter.py): each
-// check there has a twin run with one of these switched on that must fail.  0: the reference's
-// arithmetic.  Set only by those tests (orc_set_plant); never on in any other use of the oracle.
-enum {
-    PLANT_NONE = 0,
-    PLANT_INVR_SIGN = 1,    // invR(0,1) with the wrong sign
-    PLANT_ALPHA_SWAP = 2,   // alpha columns exchanged in w_delta
-    PLANT_T_SIGN = 3,       // vertex rebuild with T = +1 on the diagonal, -1 off it
-    PLANT_THETA_SWAP = 4,   // theta+ and theta- exchanged
-    PLANT_NU_ONE = 5,       // minmod against |w_delta| instead of |nu w_delta|
-    PLANT_R_TRANSPOSE = 6,  // R transposed in the characteristic back-transform
-    PLANT_NO_THETA = 7,     // positive / negative balance skipped
-    PLANT_TVB_OFF = 8       // |w_tilda| <= M r^2 branch skipped
-};
-static int g_plant = PLANT_NONE;
 
 static void R_mat(double g, double h, double u, double v, double nx, double ny, double R[3][3]) {  // swe_jacobian.hpp:462-480
     double c = std::sqrt(g * h);
@@ -1913,10 +1919,54 @@ int orc_element_integrals(int p, const double* vx, const double* vy, const doubl
     return 0;
 }
 
+// Edge integrals of one triangle whose three faces are all boundaries: the Gauss-Legendre points
+// of each edge in global coordinates (xy [3][ngp][2]), and, for values f [3][ngp] at those points,
+// ix[d] = sum over edges and points of f n_x (phi_d w |edge J|), iy[d] likewise with n_y
+// (Boundary::IntegrationPhi with the edge normal, boundary.hpp:153-207).  Returns ngp per edge.
+int orc_element_edge_integrals(int p, const double* vx, const double* vy, const double* f, double* xy, double* ix,
+                               double* iy) {
+    rkdg_mesh_desc md{};
+    double xyz[9];
+    for (int k = 0; k < 3; ++k) xyz[3 * k] = vx[k], xyz[3 * k + 1] = vy[k], xyz[3 * k + 2] = 1.0;
+    int en[3] = {0, 1, 2};
+    int be[6] = {1, 2, 2, 0, 0, 1};
+    uint8_t bt[3] = {0, 0, 0};
+    md.n_nodes = 3, md.n_elements = 1, md.node_xyz = xyz, md.elt_nodes = en;
+    md.n_bound_edges = 3, md.bound_edge_nodes = be, md.bound_edge_type = bt;
+    rkdg_params prm{};
+    prm.p = p, prm.nstages = 1, prm.order = 1, prm.dt = 1, prm.g = 9.81, prm.ic_constant = 1;
+    orc::Sim* S = orc::create(&md, &prm);
+    if (!S || S->bounds.size() != 3) {
+        delete S;
+        return -1;
+    }
+    const unsigned ndof = S->master.ndof;
+    const int ngp = (int)S->bounds[0].s.gp_global.size();
+    for (unsigned d = 0; d < ndof; ++d) ix[d] = 0.0, iy[d] = 0.0;
+    for (const orc::Boundary& B : S->bounds) {
+        const unsigned k = B.s.bound;
+        for (int g = 0; g < ngp; ++g) {
+            if (xy) xy[(k * ngp + g) * 2] = B.s.gp_global[g][0], xy[(k * ngp + g) * 2 + 1] = B.s.gp_global[g][1];
+            if (!f) continue;
+            const double fv = f[k * ngp + g];
+            for (unsigned d = 0; d < ndof; ++d) {
+                ix[d] = ix[d] + fv * B.s.normal[g][0] * B.s.int_phi_fact(g, d);
+                iy[d] = iy[d] + fv * B.s.normal[g][1] * B.s.int_phi_fact(g, d);
+            }
+        }
+    }
+    delete S;
+    return ngp;
+}
+
 // Global coordinates of the element's rule-2p points (for building f_gp in the test).
 int orc_element_gp_global(int p, const double* vx, const double* vy, double* xy) {
     orc::Shape sh;
     for (int k = 0; k < 3; ++k) sh.x[k] = vx[k], sh.y[k] = vy[k], sh.z[k] = 0;
+    if (sh.detJ() < 0) {  // as the element does (shape_straighttriangle.cpp:9-11)
+        std::swap(sh.x[0], sh.x[2]);
+        std::swap(sh.y[0], sh.y[2]);
+    }
     orc::Rule2D R = orc::dunavant(2 * (unsigned)p);
     auto g = sh.local_to_global(R.z);
     for (size_t k = 0; k < g.size(); ++k) xy[2 * k] = g[k][0], xy[2 * k + 1] = g[k][1];

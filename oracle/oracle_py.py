@@ -49,6 +49,7 @@ def load():
             "orc_stage_times": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, dp]),
             "orc_element_integrals": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp]),
             "orc_element_gp_global": (C.c_int, [C.c_int, dp, dp, dp]),
+            "orc_element_edge_integrals": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp]),
             "orc_wet_dry_single": (C.c_int, [dp, dp, dp, C.c_double, C.c_double, C.c_int, dp, ip, ip, dp]),
             "orc_set_plant": (None, [C.c_int]),
             "orc_R": (None, [C.c_double] * 6 + [dp]),
