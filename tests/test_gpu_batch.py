@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from conftest import relerr
-from dgswemv2_amd import (RKDG_FP32_BULK, RKDG_FP64_PLAIN, Batch, RkdgError, Simulation, make_params,
+from dgswemv2_amd import (RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, Batch, RkdgError, Simulation, make_params,
                           workloads as W)
 
 pytestmark = pytest.mark.gpu
@@ -43,7 +43,7 @@ def singles(cases, precision, steps):
     return res
 
 
-@pytest.mark.parametrize("precision", [RKDG_FP64_PLAIN, RKDG_FP32_BULK])
+@pytest.mark.parametrize("precision", [RKDG_FP64_PLAIN, RKDG_FP64_FAST, RKDG_FP32_BULK])
 def test_tidal_batch_equals_single_runs(precision):
     cases = tidal_instances()
     steps = 6
