@@ -58,7 +58,7 @@ def dof_stages(ne, ndof, stages):
     return ne * ndof * 3 * stages
 
 
-def survey_bytes_per_element(p, nstages, b):
+def algorithmic_bytes_per_element(p, nstages, b):
     """Algorithmic HBM bytes per element-stage: read q_s and write q_s+1 (2 S), read the first
     earlier state in the stages that combine it (r S, r = 1/2 for SSP-RK2, 2/3 for SSP-RK3), modal
     bathymetry (ndof b), |det J| and J^-1 (5 b), and 1.5 edges per element with two int32 ids,
@@ -236,7 +236,7 @@ def ensemble_bench(steps, warmup, scale):
     wall = time.perf_counter() - t0
     window_ms = b.report()["total_ms"]  # HIP events around the timed steps on the batch's stream
     ns = params[0].nstages
-    step_bytes = ns * sum(survey_bytes_per_element(i["p"], ns, 4) * i["triangles"] for i in info)
+    step_bytes = ns * sum(algorithmic_bytes_per_element(i["p"], ns, 4) * i["triangles"] for i in info)
     achieved = step_bytes / (window_ms / steps * 1e-3) / 1e9
     b.close()
     return {"instances": len(meshes), "triangles": ne, "p2_instances": sum(1 for i in info if i["p"] == 2),
@@ -477,13 +477,13 @@ def main():
         stages = args.steps * ns
         per_launch_ms = rep["window_ms"] / stages  # HIP events around the timed steps
         kern_ms = rep["stage_kernel_ms"] / max(1, rep["stages_timed"])  # per-kernel events, further window
-        bytes_survey = survey_bytes_per_element(2, ns, b) * ne
-        achieved = bytes_survey / (per_launch_ms * 1e-3) / 1e9
+        bytes_alg = algorithmic_bytes_per_element(2, ns, b) * ne
+        achieved = bytes_alg / (per_launch_ms * 1e-3) / 1e9
         forms[name] = {
             "value": dof_stages(ne, ndof, stages) / wall,
             "ms_per_step": wall / args.steps * 1e3,
             "stage_launch_ms": per_launch_ms,
-            "stage_bytes_alg": bytes_survey,
+            "stage_bytes_alg": bytes_alg,
             "stage_bytes_engine": rep["stage_bytes_alg"],
             "achieved_GBs": achieved,
             "roofline_frac": achieved / HBM_PEAK_GBS,

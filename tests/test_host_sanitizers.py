@@ -1,9 +1,9 @@
 """The library's host code and the oracle under AddressSanitizer and UndefinedBehaviorSanitizer
-(CPU only).  tests/sanitize/Makefile builds a driver from the oracle and the host sources of
+(CPU only).  tests/sanitize/Makefile builds a check program from the oracle and the host sources of
 librkdg that need no GPU (mesh generator, ADCIRC / .meta readers and writers, partitioner,
 renumbering, model builder, ensemble merge, L2 residual, modal / VTU writers) with
--fsanitize=address,undefined; the driver runs them on the reference's fixtures, generated meshes
-and malformed files.  Any sanitizer report aborts the driver with a nonzero status."""
+-fsanitize=address,undefined; it runs them on the reference's fixtures, generated meshes
+and malformed files.  Any sanitizer report makes it exit with a nonzero status."""
 import os
 import subprocess
 from pathlib import Path
