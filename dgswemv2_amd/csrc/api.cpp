@@ -88,6 +88,7 @@ struct Engine final : EngineBase {
     Real *bath = nullptr, *geom = nullptr, *egeom = nullptr, *vxy = nullptr, *gmn2 = nullptr;
     Real *sp_vol = nullptr, *sp_edge = nullptr;  // spherical correction (null: sp = 1)
     Real *famp = nullptr, *fphase = nullptr, *levee_gp = nullptr;
+    Real* con = nullptr;  // [ncon][3] frequency, forcing factor, equilibrium argument
     int *nb = nullptr, *edge = nullptr, *eforce = nullptr, *neighbor = nullptr;
     uint32_t* meta = nullptr;
     uint8_t* wet[2] = {nullptr, nullptr};
@@ -243,6 +244,9 @@ struct Engine final : EngineBase {
             !upload(neighbor, M.neighbor) || !upload(bath_vrtx, M.bath_vrtx))
             return RKDG_ERR_HIP;
         if (!upload(wet[0], M.wet0) || !upload(wet[1], M.wet0)) return RKDG_ERR_HIP;
+        std::vector<double> cons;
+        for (int c = 0; c < M.ncon; ++c) cons.push_back(M.freq[c]), cons.push_back(M.ffact[c]), cons.push_back(M.eqarg[c]);
+        if (!upload(con, cons)) return RKDG_ERR_HIP;
         if (M.prm.slope_limiting) {
             if (!upload(median, M.sl_median) || !upload(alpha, M.sl_alpha) || !upload(rsq, M.sl_rsq) ||
                 !upload(len, M.sl_len) || !upload(radius, M.sl_radius) || !upload(bath_bc, M.sl_bath_bc) ||
@@ -470,8 +474,7 @@ struct Engine final : EngineBase {
         A.levee_gp = levee_gp;
         A.sp_vol = sp_vol, A.sp_edge = sp_edge;
         A.ncon = M.ncon;
-        for (int c = 0; c < M.ncon; ++c)
-            A.freq[c] = (Real)M.freq[c], A.ffact[c] = (Real)M.ffact[c], A.eqarg[c] = (Real)M.eqarg[c];
+        A.con = con;
         A.t = st.stage_time();
         A.ramp = st.ramp;
         A.g = (Real)M.prm.g;

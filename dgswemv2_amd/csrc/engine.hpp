@@ -76,7 +76,7 @@ struct StageArgs {
     const Real* famp;        // [nforced][ncon][NE]
     const Real* fphase;      // [nforced][ncon][NE]
     int ncon;
-    Real freq[4], ffact[4], eqarg[4];
+    const Real* con;         // [ncon][3] frequency, forcing factor, equilibrium argument of each constituent
     double t;                // stage time
     double ramp;
     Real g;
@@ -494,7 +494,8 @@ RKDG_HD inline Real harmonic_forcing(const StageArgs<Real>& A, int ed, int g, do
         using std::cos;
         const Real amp = A.famp[((size_t)fr * A.ncon + c) * NE + g];
         const Real ph = A.fphase[((size_t)fr * A.ncon + c) * NE + g];
-        acc += Real(ramp) * A.ffact[c] * amp * cos(A.freq[c] * Real(t) + (A.eqarg[c] - ph) * Real(kPI) / Real(180));
+        const Real freq = A.con[3 * c], ffact = A.con[3 * c + 1], eqarg = A.con[3 * c + 2];
+        acc += Real(ramp) * ffact * amp * cos(freq * Real(t) + (eqarg - ph) * Real(kPI) / Real(180));
     }
     return acc;
 }

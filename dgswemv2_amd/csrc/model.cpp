@@ -303,7 +303,7 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
             for (int k = 0; k < 3; ++k)
                 if (btype[3 * e + k] == ty) bedges.push_back({ty, e, k});
     M.ncon = md.n_constituents;
-    if (M.ncon > 4) return "more than 4 harmonic constituents";
+    if (M.ncon < 0) return "negative constituent count";
     for (int c = 0; c < M.ncon; ++c) {
         M.freq.push_back(md.frequency[c]);
         M.ffact.push_back(md.forcing_fact[c]);

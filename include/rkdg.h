@@ -69,7 +69,7 @@ enum { RKDG_FRICTION_NONE = 0, RKDG_FRICTION_CHEZY = 1, RKDG_FRICTION_MANNING = 
  * elt_nodes and its ID is e (elements iterate in ascending ID, heterogeneous_containers.hpp:76-78).
  * Every mesh edge with one adjacent element must be listed in bound_edge_nodes with its
  * boundary code (reference: AdcircFormat::get_ibtype, adcirc_format.cpp:214).
- * Tide/flow forcing: n_constituents harmonic constituents shared by all forced nodes
+ * Tide/flow forcing: n_constituents (any number) harmonic constituents shared by all forced nodes
  * (reference .bcis data, problem_input/swe_inputs.cpp:212-…), amplitude/phase per node.
  */
 typedef struct rkdg_mesh_desc {

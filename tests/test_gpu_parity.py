@@ -401,3 +401,49 @@ def test_tidal_ramp_batch_vs_oracle(graph):
             assert b.time(k)[:2] == o.time()[:2]
     banner("ramped ensemble graph", graph, "matches oracle and single runs")
     b.close()
+
+
+# ---- many tidal constituents on the open boundary (rkdg_swe_bc_tide.hpp:70-96 sums them all)
+
+def _eight_constituent_tide(seed=8):
+    from dgswemv2_amd import BC_TIDE, Mesh
+    base = quad_mesh(0.0, 0.0, 60000.0, 40000.0, 12, 8, bc=(BC_LAND, BC_LAND, BC_LAND, BC_TIDE), bathy="fourier",
+                     bmin=5.0, bmax=200.0, n_modes=4, jitter=0.2, seed=5)
+    # angular speeds (rad/s) of M2, S2, N2, K2, K1, O1, P1, Q1
+    freq = [1.405189e-4, 1.454441e-4, 1.378797e-4, 1.458423e-4, 7.292117e-5, 6.759774e-5, 7.252295e-5, 6.495854e-5]
+    rng = np.random.default_rng(seed)
+    nv, nc = len(base.node_xyz), len(freq)
+    mesh = Mesh(base.node_xyz, base.elt_nodes, base.bound_edges, base.bound_types, frequency=freq,
+                forcing_fact=rng.uniform(0.9, 1.1, nc), equilib_arg=rng.uniform(0.0, 360.0, nc),
+                node_amplitude=rng.uniform(0.05, 0.5, (nv, nc)), node_phase=rng.uniform(0.0, 360.0, (nv, nc)))
+    dt = 0.1 * mesh.min_edge_length() / (np.sqrt(9.81 * 200.0) * 5)
+    prm = make_params(p=2, nstages=3, order=3, dt=dt, g=9.81, ic_constant=True)
+    return mesh, prm
+
+
+def test_tide_eight_constituents_vs_oracle():
+    mesh, prm = _eight_constituent_tide()
+    banner("tide with 8 constituents ne", mesh.n_elements)
+    o = OracleSim(mesh, prm)
+    g = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    f = Simulation(mesh, prm, RKDG_FP64_FAST)
+    s = Simulation(mesh, prm, RKDG_FP32_BULK)
+    worst, exact = 0.0, True
+    for i in range(10):
+        o.step(1), g.step(1), f.step(1), s.step(1)
+        so, sg = o.state()["q"], g.state()["q"]
+        r = max(relerr(sg, so))
+        worst, exact = max(worst, r), exact and bool(np.array_equal(sg, so))
+        assert r <= TOL64, (i, r)
+    rf, rs = max(relerr(f.state()["q"], so)), max(relerr(s.state()["q"], so))
+    banner("8 constituents: plain worst", worst, "bit-exact", exact, "fast", rf, "fp32", rs)
+    assert rf <= TOL64 and rs <= 1e-5
+    # every constituent is felt: dropping the last one changes the run
+    from dgswemv2_amd import Mesh
+    m7 = Mesh(mesh.node_xyz, mesh.elt_nodes, mesh.bound_edges, mesh.bound_types, frequency=mesh.frequency[:7],
+              forcing_fact=mesh.forcing_fact[:7], equilib_arg=mesh.equilib_arg[:7],
+              node_amplitude=np.ascontiguousarray(mesh.node_amplitude.reshape(-1, 8)[:, :7]),
+              node_phase=np.ascontiguousarray(mesh.node_phase.reshape(-1, 8)[:, :7]))
+    u = Simulation(m7, prm, RKDG_FP64_PLAIN)
+    u.step(10)
+    assert max(relerr(u.state()["q"], so)) > 1e-3
