@@ -151,37 +151,47 @@ def cpu_baseline(runs=3, steps_per_run=4):
 
 
 def copy_bandwidth(mib=2048, reps=10):
-    """Measured HBM bandwidth of a device-to-device copy of `mib` MiB (read + write bytes per
-    second over `reps` copies): the achievable peak beside 8 TB/s.  Measured in a child process:
-    torch brings its own HIP runtime, which cannot start in a process where librkdg.so's runtime
-    already holds the device."""
+    """Measured HBM bandwidth (GB/s, bytes read + written per second over `reps` passes over
+    `mib` MiB arrays) of three streaming kernels: a device-to-device copy ("copy", one read per
+    write), an elementwise sum of two arrays ("add", two reads per write, near the stage kernel's
+    mix of three) and a full reduction ("read", reads only): the achievable rates beside 8 TB/s.
+    Measured in a child process: torch brings its own HIP runtime, which cannot start in a
+    process where librkdg.so's runtime already holds the device."""
     import subprocess
     r = subprocess.run([sys.executable, str(Path(__file__).resolve()), "--copy-gbs", str(mib), str(reps)],
                        capture_output=True, text=True, timeout=300)
     if r.returncode != 0:
         raise RuntimeError("copy bandwidth probe failed: " + r.stderr[-2000:])
-    return float(r.stdout.strip().splitlines()[-1])
+    return json.loads(r.stdout.strip().splitlines()[-1])
 
 
 def copy_gbs_child(mib, reps):
-    """The copy measurement of copy_bandwidth: torch copies on its own stream, timed by HIP events."""
+    """The measurements of copy_bandwidth: torch kernels on torch's stream, timed by HIP events;
+    returns the JSON text of {"copy", "add", "read"} in GB/s."""
     import torch
     n = mib * (1 << 20) // 4
     a = torch.ones(n, dtype=torch.float32, device="cuda")
     b = torch.empty_like(a)
-    for _ in range(3):
-        b.copy_(a)
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        b.copy_(a)
-    e1.record()
-    e1.synchronize()
-    ms = e0.elapsed_time(e1)
-    gbs = 2.0 * n * 4 * reps / (ms * 1e-3) / 1e9
-    del a, b
+    c = torch.empty_like(a)
+    s = torch.empty((), dtype=torch.float32, device="cuda")
+
+    def rate(fn, nbytes):
+        for _ in range(3):
+            fn()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return nbytes * reps / (e0.elapsed_time(e1) * 1e-3) / 1e9
+
+    out = {"copy": rate(lambda: b.copy_(a), 2.0 * n * 4),
+           "add": rate(lambda: torch.add(a, b, out=c), 3.0 * n * 4),
+           "read": rate(lambda: torch.sum(a, dim=0, out=s), 1.0 * n * 4)}
+    del a, b, c, s
     torch.cuda.empty_cache()
-    return gbs
+    return json.dumps(out)
 
 
 def gate_probe(steps, scale):
@@ -523,8 +533,9 @@ def main():
         except Exception:
             traffic = None
 
-    copy_gbs = copy_bandwidth()
-    log("device copy", round(copy_gbs, 1), "GB/s")
+    stream = copy_bandwidth()
+    copy_gbs = stream["copy"]
+    log("device streams (GB/s)", json.dumps({k: round(v, 1) for k, v in stream.items()}))
     ens = None if args.no_ensemble else ensemble_bench(args.steps, args.warmup, args.ensemble_scale)
     if ens:
         log("ensemble", json.dumps(ens))
@@ -559,7 +570,9 @@ def main():
                      "bytes_per_launch": h["stage_bytes_alg"], "bytes_per_element_stage": 520.0,
                      "engine_bytes_per_launch": h["stage_bytes_engine"],
                      "time_per_launch_ms": h["stage_launch_ms"], "measured_copy_GBs": copy_gbs,
-                     "frac_of_measured_copy": h["achieved_GBs"] / copy_gbs},
+                     "frac_of_measured_copy": h["achieved_GBs"] / copy_gbs,
+                     "measured_add_GBs": stream["add"], "measured_read_GBs": stream["read"],
+                     "frac_of_measured_add": h["achieved_GBs"] / stream["add"]},
         "cpu_baseline": cpu,
         "ensemble": ens,
         "learned": learned,

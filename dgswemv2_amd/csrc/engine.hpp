@@ -157,12 +157,12 @@ RKDG_HD inline T ldc(const T* base, int n, int c, int i) {
 #endif
 }
 
-// FP64 columns c = 0..ncol-1 of an element-fastest array for the 64 consecutive elements
-// e0..e0+63 of a wave, copied HBM -> LDS by direct-to-LDS buffer loads (no registers held while
-// they are in flight): column c lands as 64 doubles at dst[64 c ..].  Every lane of the wave must
-// execute it: lane j fetches dword j of each 256-byte half of the column slice.  Elements past
-// n - 1 read element n - 1 (their values are never used).  The data may be read only after
-// lds_columns_wait().
+// Columns c = 0..ncol-1 of an element-fastest array for the 64 consecutive elements e0..e0+63 of
+// a wave, copied HBM -> LDS by direct-to-LDS buffer loads (no registers held while they are in
+// flight): column c lands as 64 values at dst[64 c ..].  Every lane of the wave must execute it.
+// FP64: lane j fetches dword j of each 256-byte half of the column slice; FP32: lane j fetches
+// element e0 + j.  Elements past n - 1 read element n - 1 (their values are never used).  The
+// data may be read only after lds_columns_wait().
 RKDG_HD inline void lds_columns_fetch(const double* base, int n, int ncol, int e0, double* dst) {
 #ifdef __HIP_DEVICE_COMPILE__
     const int lane = (int)__lane_id();
@@ -176,6 +176,21 @@ RKDG_HD inline void lds_columns_fetch(const double* base, int n, int ncol, int e
                                                  soff, 0, 0);
         __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(dst + 64 * c + 32), 4,
                                                  vb, soff, 0, 0);
+    }
+#else
+    (void)base, (void)n, (void)ncol, (void)e0, (void)dst;
+#endif
+}
+
+RKDG_HD inline void lds_columns_fetch(const float* base, int n, int ncol, int e0, float* dst) {
+#ifdef __HIP_DEVICE_COMPILE__
+    const int lane = (int)__lane_id();
+    const int ea = e0 + lane < n ? e0 + lane : n - 1;
+    const __amdgpu_buffer_rsrc_t r = rkdg_rsrc(base);
+    for (int c = 0; c < ncol; ++c) {
+        const int soff = (int)((unsigned)c * (unsigned)n * 4u);  // below 2^32 (model builder)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)(dst + 64 * c), 4, ea * 4,
+                                                 soff, 0, 0);
     }
 #else
     (void)base, (void)n, (void)ncol, (void)e0, (void)dst;
@@ -1093,24 +1108,11 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     int nbv[3], ed[3];
 #pragma unroll
     for (int f = 0; f < 3; ++f) nbv[f] = ldc(A.nb, ne, f, e), ed[f] = ldc(A.edge, ne, f, e);
-    // FP32: the first earlier state of the SSP-RK combination (q0 in every stage of SSP-RK2/RK3
-    // that has one) is loaded here with the element's own modes, so its latency hides behind the
-    // volume integral instead of stalling the update at the end (18 registers, within the budget
-    // of three waves per SIMD).  FP64 has no registers for it: the kernel fetches it into LDS
-    // (t0_lds) when the stage starts.
-    constexpr bool kTerm0 = sizeof(Real) == 4;
-    Real t0[3][ND];
-    if (kTerm0 && A.nterm > 0) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-#pragma unroll
-            for (int d = 0; d < ND; ++d) t0[k][d] = ldc(A.term_q[0], ne, k * ND + d, e);
-    }
-    // FP32 loads neighbour modes one face ahead (face 0's before the volume integral, face f+1's
-    // while face f computes; two faces' worth of registers live at most) and the edge geometry up
-    // front.  FP64 has no registers to spare: it loads both at the face itself (occupancy 2
-    // instead of 1).
-    constexpr bool kAhead = sizeof(Real) == 4;
+    // The first earlier state of the SSP-RK combination (q0, read by the update at the end) is
+    // fetched into LDS (t0_lds) when the stage starts, so it holds no registers while in flight.
+    // Neighbour modes and edge geometry are loaded at the face itself, which keeps one face's
+    // worth of neighbour registers live: FP32 113 VGPRs (4 waves per SIMD; 155 and 3 waves when
+    // it held q0 and the next face's modes in registers), FP64 215 (2 waves).
     auto load_nb = [&](int f, Real (&qm)[3][ND], Real (&bm)[ND]) {
         const int n = nbv[f] >= 0 ? nbv[f] : e;  // boundary faces read their own (cached) modes
 #pragma unroll
@@ -1121,13 +1123,6 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
         }
     };
     Real qn_m[3][ND], bn_m[ND];
-    if (kAhead) load_nb(0, qn_m, bn_m);
-    Real nx[3], ny[3], sJ[3];
-    if (kAhead)
-#pragma unroll
-        for (int f = 0; f < 3; ++f)
-            nx[f] = ldc(A.egeom, A.nedge, 0, ed[f]), ny[f] = ldc(A.egeom, A.nedge, 1, ed[f]),
-            sJ[f] = ldc(A.egeom, A.nedge, 2, ed[f]);
     const bool wet = WD ? (A.wet_in[e] != 0) : true;
     bool wet_nb[3];
 #pragma unroll
@@ -1224,13 +1219,11 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     // flux and lambda see identical operands on both sides and stay exactly antisymmetric.
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        Real qn_next[3][ND], bn_next[ND];
-        if (kAhead && f < 2) load_nb(f + 1, qn_next, bn_next);
-        if (!kAhead) load_nb(f, qn_m, bn_m);
+        load_nb(f, qn_m, bn_m);
         if (nbv[f] >= 0 && !((meta >> (12 + f)) & 1) && (wet || wet_nb[f])) {
-            const Real fnx = kAhead ? nx[f] : ldc(A.egeom, A.nedge, 0, ed[f]);
-            const Real fny = kAhead ? ny[f] : ldc(A.egeom, A.nedge, 1, ed[f]);
-            const Real fsJ = kAhead ? sJ[f] : ldc(A.egeom, A.nedge, 2, ed[f]);
+            const Real fnx = ldc(A.egeom, A.nedge, 0, ed[f]);
+            const Real fny = ldc(A.egeom, A.nedge, 1, ed[f]);
+            const Real fsJ = ldc(A.egeom, A.nedge, 2, ed[f]);
             const int fn = (meta >> (4 * f + 2)) & 3;
             const Real* pn = phie_lds + fn * ND * NE;
             const bool is_in = e < nbv[f];
@@ -1282,14 +1275,6 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
                     for (int g = 0; g < NE; ++g) r = fmac_tab(F[k][g], -C::phiew[f * ND + d][g], r);
                     rhs[k][d] = r;
                 }
-        }
-        if (kAhead && f < 2) {
-#pragma unroll
-            for (int d = 0; d < ND; ++d) {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) qn_m[k][d] = qn_next[k][d];
-                bn_m[d] = bn_next[d];
-            }
         }
     }
 
@@ -1359,9 +1344,9 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
             const Real* pf = phie_lds + f * ND * NE;
             const Real* pw = phiew_lds + f * ND * NE;
             const int fed = f == 0 ? ed[0] : (f == 1 ? ed[1] : ed[2]);
-            const Real fnx = kAhead ? (f == 0 ? nx[0] : (f == 1 ? nx[1] : nx[2])) : ldc(A.egeom, A.nedge, 0, fed);
-            const Real fny = kAhead ? (f == 0 ? ny[0] : (f == 1 ? ny[1] : ny[2])) : ldc(A.egeom, A.nedge, 1, fed);
-            const Real fsJ = kAhead ? (f == 0 ? sJ[0] : (f == 1 ? sJ[1] : sJ[2])) : ldc(A.egeom, A.nedge, 2, fed);
+            const Real fnx = ldc(A.egeom, A.nedge, 0, fed);
+            const Real fny = ldc(A.egeom, A.nedge, 1, fed);
+            const Real fsJ = ldc(A.egeom, A.nedge, 2, fed);
             Real F[3][NE];
 #pragma unroll
             for (int g = 0; g < NE; ++g) {
@@ -1422,8 +1407,7 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
             if (so) stc(so, ne, k * ND + d, e, sol);
             Real acc = ca * q[k][d] + cc * sol;
             if (nterm > 0) {
-                Real term = ta0 * (kTerm0 ? t0[k][d]
-                                   : t0_lds ? t0_lds[64 * (k * ND + d) + lane_index()] : ldc(tq0, ne, k * ND + d, e));
+                Real term = ta0 * (t0_lds ? t0_lds[64 * (k * ND + d) + lane_index()] : ldc(tq0, ne, k * ND + d, e));
                 if (ts0) term += tc0 * ldc(ts0, ne, k * ND + d, e);
                 acc += term;
                 for (int i = 1; i < nterm; ++i) {
