@@ -890,8 +890,8 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element(const StageArgs<Real>& A, co
 // per-element D matrices; volume, source and lifts accumulate unscaled and M^-1/|det J| is applied
 // once; divisions become reciprocals; faces run in local-face order so the own-face tables are
 // compile-time constants (the neighbour's face table comes from LDS).  Both neighbours still
-// evaluate an interface flux with identical operands (in side = lower element ID), so fluxes
-// stay exactly antisymmetric and mass is conserved to roundoff.
+// evaluate each interface flux, each from its own side; the mass flux stays exactly
+// antisymmetric, so mass is conserved to roundoff.
 
 // Fast-form elementary operations.  FP32: hardware reciprocal / square root / exp2-log2
 // (about 1 ulp).  FP64: hardware reciprocal refined by two Newton steps, square root from the
@@ -1002,16 +1002,19 @@ RKDG_HD inline void llf_rel(Real grav, const Real* qi, const Real* qx, Real bath
     F[2] = Real(0.5) * (v_in * fi0 + v_ex * fe0 + (pi + px) * ny + lam * (qi[2] - qx[2]));
 }
 
-// Interface flux of one side in the relative-pressure form: pi/px as for llf_rel, r the
-// caller's hydrostatic reference at this point.  Returns the caller's flux (w.r.t. its outward
-// normal) with momentum relative to r.  The wet/dry fixes of rkdg_swe_is_internal.hpp:47-90
-// (rare) are applied to the plain flux, rebuilt as F_rel + r n.
+// Interface flux of one side in the relative-pressure form: pi/px as for llf_rel, bl the
+// bathymetry the LLF flux uses (the in side's), bi/bx each side's own (for the wet/dry fixes), r
+// the caller's hydrostatic reference at this point.  Returns the flux of the side given first
+// (want_in) or second, w.r.t. that side's outward normal, with momentum relative to r.  The
+// wet/dry fixes of rkdg_swe_is_internal.hpp:47-90 (rare) are applied to the plain flux, rebuilt as
+// F_rel + r n; they depend only on the sign of the mass flux and on which side is dry, so they
+// give each side the same flux whichever side is passed first.
 template <typename Real, bool WD, bool SPH = false>
-RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* qx, Real pi, Real px, Real bi,
+RKDG_HD inline void interface_flux_fast(Real grav, const Real* qi, const Real* qx, Real pi, Real px, Real bl, Real bi,
                                         Real bx, Real nx, Real ny, bool wet_in, bool wet_ex, bool want_in, Real r,
                                         Real* F, Real sp = Real(1)) {
     Real Fi[3];
-    llf_rel<Real, SPH>(grav, qi, qx, bi, pi, px, nx, ny, Fi, sp);
+    llf_rel<Real, SPH>(grav, qi, qx, bl, pi, px, nx, ny, Fi, sp);
     if (WD && ((Fi[0] > Real(1e-12) && !(wet_in && wet_ex)) || (Fi[0] < -Real(1e-12) && !(wet_in && wet_ex)))) {
         Fi[1] += r * nx, Fi[2] += r * ny;
         Real Fe[3] = {-Fi[0], -Fi[1], -Fi[2]};
@@ -1215,8 +1218,14 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
     }
 
     // ---- interior faces (local face order; own tables compile-time, the neighbour's from LDS).
-    // Both sides form the surface trace as zc + (sum of modes 1..) in the same order, so the mass
-    // flux and lambda see identical operands on both sides and stay exactly antisymmetric.
+    // Each side evaluates the LLF flux from its own side: its own trace first, its outward normal,
+    // the in side's bathymetry.  Both sides form the surface trace as zc + (sum of modes 1..) in
+    // the same order, so the two evaluations see the same traces, normals that are exact
+    // negations and the same lambda; the mass flux, a sum of two normal fluxes and lambda times a
+    // trace difference, comes out exactly negated on the other side (mass is conserved to
+    // roundoff), the momentum components agree to rounding.  Builds with the closed-form problem
+    // functions (CF) carry the spherical correction, which scales the in side's normal flux only:
+    // they keep the in/ex roles (the lower element ID evaluates as the in side).
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
         load_nb(f, qn_m, bn_m);
@@ -1229,6 +1238,7 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
             const bool is_in = e < nbv[f];
             const Real zcn = qn_m[0][0] * PHI0;
             const Real dzr = zcn - zr;
+            const Real mx = is_in ? fnx : -fnx, my = is_in ? fny : -fny;  // this element's outward normal
             Real F[3][NE];
 #pragma unroll
             for (int g = 0; g < NE; ++g) {
@@ -1255,11 +1265,14 @@ RKDG_HD RKDG_FORCE_INLINE inline void stage_element_fast(const StageArgs<Real>& 
                 const Real rref = grav * zr * (Real(0.5) * zr + bo);
                 const Real sp = sph ? ldc(A.sp_edge, A.nedge, is_in ? g : gm, ed[f]) : Real(1);
                 Real Fg[3];
-                if (is_in)
-                    interface_flux_fast<Real, WD, CF>(grav, own, nbs, po, pnb, bo, bn, fnx, fny, wet, wet_nb[f], true, rref, Fg,
-                                                      sp);
+                if (!CF)  // from this element's side: no per-lane exchange of the two sides' operands
+                    interface_flux_fast<Real, WD, CF>(grav, own, nbs, po, pnb, B, bo, bn, mx, my, wet, wet_nb[f], true, rref,
+                                                      Fg);
+                else if (is_in)
+                    interface_flux_fast<Real, WD, CF>(grav, own, nbs, po, pnb, bo, bo, bn, fnx, fny, wet, wet_nb[f], true, rref,
+                                                      Fg, sp);
                 else
-                    interface_flux_fast<Real, WD, CF>(grav, nbs, own, pnb, po, bn, bo, fnx, fny, wet_nb[f], wet, false, rref,
+                    interface_flux_fast<Real, WD, CF>(grav, nbs, own, pnb, po, bn, bn, bo, fnx, fny, wet_nb[f], wet, false, rref,
                                                       Fg, sp);
                 F[0][g] = Fg[0], F[1][g] = Fg[1], F[2][g] = Fg[2];
             }
