@@ -29,6 +29,9 @@ thread_local std::string g_create_error;
 namespace rkdg {
 // Sets the message rkdg_create_error() returns (also used by the mesh reader).
 void set_create_error(const std::string& msg) { g_create_error = msg; }
+
+// Polynomial orders the kernels are built for; the reference takes any p >= 0.
+static bool order_built(int p) { return p >= 1 && p <= 3; }
 }  // namespace rkdg
 
 namespace {
@@ -832,7 +835,7 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
     }
     if (!msg.empty()) {
         g_create_error = msg;
-        return RKDG_ERR_ARG;
+        return params->p >= 0 && !rkdg::order_built(params->p) ? RKDG_ERR_UNSUPPORTED : RKDG_ERR_ARG;
     }
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
@@ -1203,7 +1206,7 @@ rkdg_status rkdg_create_batch(int32_t n, const rkdg_mesh_desc* meshes, const rkd
         if (msg.empty() && !(params[i].dt > 0)) msg = "dt must be positive";
         if (!msg.empty()) {
             g_create_error = "instance " + std::to_string(i) + ": " + msg;
-            return RKDG_ERR_ARG;
+            return params[i].p >= 0 && !rkdg::order_built(params[i].p) ? RKDG_ERR_UNSUPPORTED : RKDG_ERR_ARG;
         }
         by_p[params[i].p].push_back(i);
     }

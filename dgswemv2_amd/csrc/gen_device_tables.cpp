@@ -1,4 +1,4 @@
-// Build-time generator of device_tables.h: the reference-element tables of p = 1, 2 as
+// Build-time generator of device_tables.h: the reference-element tables of p = 1, 2, 3 as
 // compile-time constants (exact hex-float literals of the host FP64 tables from tables.cpp),
 // so the kernels read them as instruction constants instead of memory.
 #include <cstdio>
@@ -62,7 +62,7 @@ static void emit_type(int p, const char* type) {
 int main() {
     std::printf("// Generated at build time by gen_device_tables (from tables.cpp) -- do not edit.\n");
     std::printf("#pragma once\nnamespace rkdg {\ntemplate <typename Real, int P>\nstruct ConstTables;\n\n");
-    for (int p = 1; p <= 2; ++p) {
+    for (int p = 1; p <= 3; ++p) {
         emit_type(p, "double");
         emit_type(p, "float");
     }

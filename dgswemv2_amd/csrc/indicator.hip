@@ -305,6 +305,11 @@ hipError_t launch_gate(int p, const GateArgs<Real>& G, bool mlp_on, hipStream_t 
             hipLaunchKernelGGL((gate_kernel<Real, 2, true>), grid, block, 0, s, G, ntiles);
         else
             hipLaunchKernelGGL((gate_kernel<Real, 2, false>), grid, block, 0, s, G, ntiles);
+    } else if (p == 3) {
+        if (mlp_on)
+            hipLaunchKernelGGL((gate_kernel<Real, 3, true>), grid, block, 0, s, G, ntiles);
+        else
+            hipLaunchKernelGGL((gate_kernel<Real, 3, false>), grid, block, 0, s, G, ntiles);
     } else {
         return hipErrorInvalidValue;
     }

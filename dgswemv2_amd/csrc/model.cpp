@@ -120,7 +120,7 @@ std::vector<double> shape_psi(const std::vector<std::array<double, 2>>& pts) {
 std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M, bool reference_face_order) {
     M.reference_face_order = reference_face_order;
     if (md.n_elements <= 0 || md.n_nodes <= 0 || !md.node_xyz || !md.elt_nodes) return "empty mesh";
-    if (prm.p < 1 || prm.p > 2) return "polynomial order outside the device-built set {1, 2}";
+    if (prm.p < 1 || prm.p > 3) return "polynomial order outside the device-built set {1, 2, 3}";
     // the kernels address every element-fastest array with 32-bit byte offsets
     if ((size_t)md.n_elements * 3 * 6 * sizeof(double) >= ((size_t)1 << 32))
         return "mesh too large: state arrays exceed 4 GiB of 32-bit buffer offsets";
@@ -408,8 +408,8 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
     M.gmn2.assign(ne, 0.0);
     // per element: v (r x 3 nodal or r x ng point values) -> * psi -> * int_phi_fact -> * M^-1,
     // each product summed from its first term (the order of the reference's Eigen products);
-    // fixed-size arrays (p <= 2: ndof, ngp <= 6)
-    constexpr int kMax = 6;
+    // fixed-size arrays (p <= 3: ndof <= 10, ngp <= 12)
+    constexpr int kMax = 12;
     if (nd > kMax || ng > kMax) return "element tables larger than the device-built set";
     double ipf[kMax * kMax], minv[kMax];
     auto project = [&](const double* v, int rows, bool nodal, double* out) {  // out[rows][nd]

@@ -32,7 +32,7 @@ typedef enum {
     RKDG_ERR_ARG = 1,         /* invalid argument / mesh (reference: std::logic_error at setup) */
     RKDG_ERR_HIP = 2,         /* a HIP runtime call failed */
     RKDG_ERR_NAN = 3,         /* NaN in the state (reference: scrutinize_solution -> abort()) */
-    RKDG_ERR_UNSUPPORTED = 4, /* configuration not built (e.g. p outside 1..2 on the device) */
+    RKDG_ERR_UNSUPPORTED = 4, /* configuration not built (e.g. p outside 1..3 on the device) */
     RKDG_ERR_NO_DEVICE = 5    /* no HIP device visible */
 } rkdg_status;
 
@@ -101,7 +101,7 @@ typedef struct rkdg_mesh_desc {
  * problem_input/swe_inputs.hpp:126-155, and the globals set by
  * problem_preprocessor/swe_pre_init_prob_params.hpp:5-47). */
 typedef struct rkdg_params {
-    int32_t p;               /* polynomial order */
+    int32_t p;               /* polynomial order: 1, 2 or 3 */
     int32_t nstages;         /* SSP-RK stages */
     int32_t order;           /* SSP-RK order */
     double dt;
@@ -152,7 +152,9 @@ typedef struct rkdg_perf_report {
  *   SWE::RKDG::Problem::create_interfaces/create_boundaries (rkdg_swe_problem.hpp:57-113),
  *   preprocessor_serial + initialize_data_serial (kernels_preprocessor/rkdg_swe_pre_serial.hpp:8-16,
  *   problem_preprocessor/swe_pre_init_data.hpp:8-327) and the ESSPRKStepper ctor
- *   (simulation/stepper/explicit_ssp_rk_stepper.cpp:3-14). */
+ *   (simulation/stepper/explicit_ssp_rk_stepper.cpp:3-14).
+ * RKDG_ERR_UNSUPPORTED for a polynomial order the kernels are not built for (p = 0 or p > 3),
+ * RKDG_ERR_ARG for any other invalid description (message from rkdg_create_error). */
 rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, rkdg_precision precision,
                         rkdg_ctx** out);
 void rkdg_destroy(rkdg_ctx* ctx);

@@ -27,7 +27,7 @@ def banner(*a):
 
 def tidal_instances():
     out = []
-    for scale, p, seed in ((0.03, 1, 3), (0.05, 2, 5), (0.02, 2, 7), (0.04, 1, 9)):
+    for scale, p, seed in ((0.03, 1, 3), (0.05, 2, 5), (0.02, 2, 7), (0.04, 1, 9), (0.03, 3, 11)):
         mesh, prm = W.tidal(scale=scale, p=p, seed=seed)
         out.append((mesh, prm))
     return out

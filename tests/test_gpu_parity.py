@@ -5,9 +5,12 @@ and troubled flags, on the same meshes and inputs.  The FP64 kernels follow the 
 operation order, so most workloads match bit for bit; libm transcendentals (tide cos,
 manufactured sin/cos/exp, Manning cbrt/hypot) may differ by an ulp on the device.
 FP64-fast form (fused multiply-adds, reordered sums, Newton reciprocals): per-step relative
-L2 <= 1e-12 against the oracle and identical wet flags; the limiter's write-back test
-(relative change > 1e-12 of the vertex values, swe_CS_slope_limiter.hpp:181-187) is decided by
-roundoff on near-zero momentum, so its flags may differ in a handful of elements (counted).
+L2 <= 1e-12 against the oracle (3e-12 at p = 3, where rounding is larger) and identical wet
+flags; the limiter's write-back test (relative change > 1e-12 of the vertex values,
+swe_CS_slope_limiter.hpp:181-187) is decided by roundoff in near-dry elements, so its flags may
+differ in a handful of elements (counted).  At p >= 2 such a flip also zeroes an element's
+higher modes on one side only, and a step can then differ by 1e-3-1e-2 in momentum; the
+limiter workloads run the fast form at p = 1 here.
 FP32-bulk form: relative L2 against the FP64 oracle stated and bounded per test.
 The 1M-triangle tidal workload and the 500k-triangle dam break are also compared with the oracle
 for a few steps at their full sizes; over longer runs at those sizes the tests check size-independent
@@ -99,6 +102,32 @@ def test_dambreak_full_size_vs_oracle():
     compare_run("dambreak", 4, scale=1.0)
 
 
+@pytest.mark.parametrize("name,kw,steps", [
+    ("tidal", dict(scale=0.05, p=3), 10),
+    ("bowl", dict(scale=0.25, p=3), 30),
+    ("dambreak", dict(scale=0.05, p=3), 30),
+    ("manufactured", dict(scale=0.25, p=3, nstages=3, order=3, dt=2e-4), 20),
+])
+def test_p3_plain_vs_oracle(name, kw, steps):
+    # p = 3 (10 modes, 12-point Dunavant rule, 4 Gauss-Legendre points per edge): the same bar as
+    # p = 1, 2, wet / limited / troubled flags included
+    compare_run(name, steps, **kw)
+
+
+def test_p3_tidal_per_stage():
+    worst, exact = compare_run("tidal", 3, per_stage=True, scale=0.05, p=3)
+    assert exact
+
+
+def test_unbuilt_order_is_unsupported():
+    for p in (0, 4):
+        mesh, prm = W.tidal(scale=0.02, p=2)
+        prm.p = p
+        with pytest.raises(RkdgError) as ei:
+            Simulation(mesh, prm)
+        assert ei.value.status == N.RKDG_ERR_UNSUPPORTED, (p, ei.value)
+
+
 def test_tidal_p2_rk3_steps():
     _, exact = compare_run("tidal", 10, scale=0.06)
 
@@ -107,15 +136,22 @@ def test_dambreak_p1():
     compare_run("dambreak", 30, scale=0.06)
 
 
-@pytest.mark.parametrize("name,kw,steps,max_flip_frac", [
-    ("manufactured", dict(scale=0.25), 20, 0.0),
-    ("bowl", dict(scale=0.25), 20, 0.02),
-    ("tidal", dict(scale=0.06), 10, 0.0),
-    ("dambreak", dict(scale=0.06), 20, 0.02),
+@pytest.mark.parametrize("name,kw,steps,max_flip_frac,tol", [
+    ("manufactured", dict(scale=0.25), 20, 0.0, TOL64),
+    ("bowl", dict(scale=0.25), 20, 0.02, TOL64),
+    ("tidal", dict(scale=0.06), 10, 0.0, TOL64),
+    ("dambreak", dict(scale=0.06), 20, 0.02, TOL64),
+    ("tidal", dict(scale=0.05, p=3), 10, 0.0, TOL64),
+    # p = 3: the fast form's per-step rounding against the reference order grows with the order
+    # (this dam break: worst 1e-13 at p = 1, 3e-13 at p = 2, 1.1e-12 at p = 3, no flag differing)
+    ("dambreak", dict(scale=0.05, p=3), 20, 0.0, 3e-12),
 ])
-def test_fp64_fast_against_oracle(name, kw, steps, max_flip_frac):
+def test_fp64_fast_against_oracle(name, kw, steps, max_flip_frac, tol):
     # The per-step L2 bound below is taken after the limiter, so it covers the effect of any
-    # flipped write-back decision on the state; the flip count itself is reported.
+    # flipped write-back decision on the state; the flip count itself is reported.  At p = 1 a
+    # flipped write-back moves the linear modes by about 1e-12 of their size; at p >= 2 it also
+    # zeroes (or keeps) the element's higher modes, so the workloads above that flip decisions
+    # (the bowl) are run at p = 1 here.
     mesh, prm = W.WORKLOADS[name](**kw)
     banner(name, kw, "fp64-fast ne", mesh.n_elements, "steps", steps)
     o = OracleSim(mesh, prm)
@@ -128,7 +164,7 @@ def test_fp64_fast_against_oracle(name, kw, steps, max_flip_frac):
         so, sg = o.state(), g.state()
         r = max(relerr(sg["q"], so["q"]))
         worst = max(worst, r)
-        assert r <= TOL64, (name, i, r)
+        assert r <= tol, (name, i, r)
         assert np.array_equal(sg["wet"], so["wet"]), (name, i)
         flips = max(flips, int((sg["limited"] != so["limited"]).sum()))
     banner(name, "fp64-fast worst per-step rel L2", worst, "max limited-flag flips", flips)
@@ -148,6 +184,7 @@ def test_full_size_tidal_fp64_fast_vs_plain():
 @pytest.mark.parametrize("name,kw,steps,tol", [
     ("manufactured", dict(scale=0.25), 20, 1e-5),
     ("tidal", dict(scale=0.04), 10, 1e-5),
+    ("tidal", dict(scale=0.05, p=3), 10, 1e-5),
 ])
 def test_fp32_bulk_against_fp64_oracle(name, kw, steps, tol):
     mesh, prm = W.WORKLOADS[name](**kw)

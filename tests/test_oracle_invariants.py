@@ -5,7 +5,8 @@ exists to compare against -- the reference's CPU path does not build here).
   mass: with land boundaries total mass changes only by roundoff
   manufactured solution: second-order convergence when dt ~ h^2 (the reference's drk = 0
     stage time makes forcing first order in dt, so dt must shrink like h^2); p = 2 third order
-    with dt ~ h^3, and first order with dt ~ h (the drk = 0 quirk seen in whole runs)
+    with dt ~ h^3, p = 3 fourth order with dt ~ h^4, and first order with dt ~ h (the drk = 0
+    quirk seen in whole runs)
   limiter: cell means are preserved; a piecewise-constant field is never limited
   OpenMP (coloured interfaces) agrees with serial to roundoff
 """
@@ -25,11 +26,11 @@ def mass(mesh, q):
     return float(np.sum(q[0, 0] * area))
 
 
-@pytest.mark.parametrize("p", [1, 2])
+@pytest.mark.parametrize("p", [1, 2, 3])
 def test_lake_at_rest(p):
     mesh = quad_mesh(0, 0, 20000, 20000, 12, 12, bc=(BC_LAND,) * 4, bathy="fourier", bmin=5, bmax=400, seed=7,
                      jitter=0.2)
-    prm = make_params(p=p, nstages=3 if p == 2 else 2, order=3 if p == 2 else 2, dt=1.0, ze0=0.7)
+    prm = make_params(p=p, nstages=3 if p >= 2 else 2, order=3 if p >= 2 else 2, dt=1.0, ze0=0.7)
     o = OracleSim(mesh, prm)
     q0 = o.state()["q"].copy()
     o.step(20)
@@ -77,6 +78,20 @@ def test_manufactured_convergence_p2():
     r3 = err(0.25, 5e-4, 0.02) / err(0.5, 6.25e-5, 0.02)
     r1 = err(0.25, 2e-3, 0.1) / err(0.5, 1e-3, 0.1)
     assert 7.0 < r3 < 9.0 and 1.7 < r1 < 2.5, (r3, r1)
+
+
+def test_manufactured_convergence_p3():
+    """p = 3, SSP-RK3: fourth order once dt shrinks like h^4 (error ratio 16 per halving of h; the
+    drk = 0 stage time keeps the forcing first order in dt, so dt must fall that fast)."""
+    def err(scale, dt, t_end):
+        mesh, prm = W.manufactured(scale=scale, p=3, nstages=3, order=3, dt=dt)
+        o = OracleSim(mesh, prm, threads=4)
+        o.step(int(round(t_end / dt)))
+        e = o.l2_residual()
+        o.close()
+        return e
+    r4 = err(0.25, 2e-4, 0.02) / err(0.5, 1.25e-5, 0.02)
+    assert 14.0 < r4 < 18.0, r4
 
 
 def test_manufactured_convergence_catches_planted_error():
