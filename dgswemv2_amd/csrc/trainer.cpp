@@ -11,6 +11,7 @@
 #include <numeric>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/rkdg.h"
@@ -167,15 +168,30 @@ rkdg_status rkdg_mlp_train(rkdg_mlp_trainer* t, const rkdg_train_params* prm, fl
         t->step_loss_cap = steps;
     }
     std::mt19937_64 rng(prm->seed);
-    std::vector<int> perm(n);
-    std::vector<float> sl(steps);
-    for (int ep = 0; ep < prm->epochs; ++ep) {
-        std::iota(perm.begin(), perm.end(), 0);
+    // Sample orders of consecutive epochs alternate between two host buffers: a host thread
+    // shuffles the next epoch's while this one launches and the GPU trains on the current one (the
+    // copy from the other buffer completed before the previous epoch's synchronisation; only that
+    // thread draws from the generator meanwhile, so the sequence of orders is unchanged).
+    std::vector<int> perm[2] = {std::vector<int>(n), std::vector<int>(n)};
+    auto shuffle = [&](std::vector<int>& pm) {
+        std::iota(pm.begin(), pm.end(), 0);
         for (int i = n - 1; i > 0; --i) {  // Fisher-Yates with a fixed generator
             const int j = (int)(rng() % (uint64_t)(i + 1));
-            std::swap(perm[i], perm[j]);
+            std::swap(pm[i], pm[j]);
         }
-        hipError_t e = hipMemcpyAsync(t->idx, perm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, t->stream);
+    };
+    std::vector<float> sl(steps);
+    if (prm->epochs > 0) shuffle(perm[0]);
+    for (int ep = 0; ep < prm->epochs; ++ep) {
+        const std::vector<int>& pm = perm[ep & 1];
+        hipError_t e = hipMemcpyAsync(t->idx, pm.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice, t->stream);
+        std::thread next;
+        bool shuffled = ep + 1 >= prm->epochs;
+        if (!shuffled) try {
+                next = std::thread(shuffle, std::ref(perm[(ep + 1) & 1]));
+                shuffled = true;
+            } catch (const std::exception&) {  // no thread to be had: shuffle after the launches
+            }
         for (int s = 0; s < steps && e == hipSuccess; ++s) {
             const int b0 = s * batch, B = std::min(batch, n - b0);
             const rkdg::TrainArgs T = t->args(t->X, t->cap, t->y, t->idx + b0, B, prm->pos_weight);
@@ -190,6 +206,8 @@ rkdg_status rkdg_mlp_train(rkdg_mlp_trainer* t, const rkdg_train_params* prm, fl
                                             prm->beta1, prm->beta2, prm->eps, c1, c2, true, t->step_loss + s, t->stream);
             if (e == hipSuccess) e = rkdg::launch_mlp_pack(t->theta, t->wpack, t->stream);
         }
+        if (next.joinable()) next.join();
+        if (!shuffled) shuffle(perm[(ep + 1) & 1]);
         if (e == hipSuccess) e = hipMemcpyAsync(sl.data(), t->step_loss, steps * sizeof(float), hipMemcpyDeviceToHost, t->stream);
         if (e == hipSuccess) e = hipStreamSynchronize(t->stream);
         if (e != hipSuccess) return t->fail("training step", e), RKDG_ERR_HIP;

@@ -109,7 +109,9 @@ def train(X, y, epochs=6, batch=1024, lr=2e-3, pos_weight=8.0, recall=0.9999, ma
     rng = np.random.default_rng(seed)
     perm = rng.permutation(n)
     nv = max(1, int(holdout * n))
-    va, tr = perm[:nv], perm[nv:]
+    # both subsets in sample order: the trainer reshuffles every epoch, and an ordered gather of
+    # the 36 feature rows is about twice as fast as a random one
+    va, tr = np.sort(perm[:nv]), np.sort(perm[nv:])
     Xt, yt = np.ascontiguousarray(X[:, tr]), np.ascontiguousarray(y[tr])
     t = MlpTrainer(init_weights(seed) if init is None else init, capacity=max(tr.size, nv))
     try:
