@@ -281,6 +281,20 @@ template <typename Real>
 RKDG_HD inline Real rmax(Real a, Real b) {  // std::max semantics: (a < b) ? b : a
     return (a < b) ? b : a;
 }
+
+// Depth under the wave speed's square root.  FP32: a depth the FP64 wetting-drying pass left at
+// or just above zero at a vertex can come back from the FP32 modes a rounding error below zero at
+// a trace point; it is taken as zero there (the square root would give NaN).  FP64: unchanged.
+RKDG_HD inline float depth_for_speed(float h) { return h > 0.0f ? h : 0.0f; }
+RKDG_HD inline double depth_for_speed(double h) { return h; }
+
+// Depth under a velocity's division (u = qx / h).  FP32: a dry element is flattened by the FP64
+// wetting-drying pass to the depth floor thr (about 1e-12) with zero momentum, and its depth comes
+// back from the FP32 modes within rounding of zero (ze of metres carries 1e-7), or below; it is
+// taken as the smallest normal float there, so the zero momentum gives a zero velocity (0 / 0
+// would be NaN).  FP64 keeps such depths positive (rounding 1e-16 against thr): unchanged.
+RKDG_HD inline float depth_for_div(float h) { return h > 1.17549435e-38f ? h : 1.17549435e-38f; }
+RKDG_HD inline double depth_for_div(double h) { return h; }
 template <typename Real>
 RKDG_HD inline Real rmin(Real a, Real b) {  // std::min semantics: (b < a) ? b : a
     return (b < a) ? b : a;
@@ -292,15 +306,16 @@ RKDG_HD inline void llf(Real grav, const Real* qi, const Real* qx, Real bath, Re
     using std::fabs;
     using std::sqrt;
     const Real h_in = qi[0] + bath;
-    const Real u_in = qi[1] / h_in;
-    const Real v_in = qi[2] / h_in;
+    const Real u_in = qi[1] / depth_for_div(h_in);
+    const Real v_in = qi[2] / depth_for_div(h_in);
     const Real h_ex = qx[0] + bath;
-    const Real u_ex = qx[1] / h_ex;
-    const Real v_ex = qx[2] / h_ex;
+    const Real u_ex = qx[1] / depth_for_div(h_ex);
+    const Real v_ex = qx[2] / depth_for_div(h_ex);
     const Real un_in = u_in * nx + v_in * ny;
     const Real un_ex = u_ex * nx + v_ex * ny;
     const Real spc = (nx * sp) * (nx * sp) + ny * ny;
-    const Real lam = rmax(fabs(un_in) + sqrt(grav * h_in * spc), fabs(un_ex) + sqrt(grav * h_ex * spc));
+    const Real lam = rmax(fabs(un_in) + sqrt(grav * depth_for_speed(h_in) * spc),
+                          fabs(un_ex) + sqrt(grav * depth_for_speed(h_ex) * spc));
     const Real uuh_in = u_in * qi[1], vvh_in = v_in * qi[2], uvh_in = u_in * qi[2];
     const Real pe_in = grav * ((qi[0] * qi[0]) / 2 + qi[0] * bath);
     const Real uuh_ex = u_ex * qx[1], vvh_ex = v_ex * qx[2], uvh_ex = u_ex * qx[2];
@@ -405,15 +420,17 @@ RKDG_HD inline void levee_flux(Real grav, bool is_in, const Real* qo, const Real
 // ---------------------------------------------------------------------------------------
 // Wetting-drying of one element on its linear (vertex) state, in FP64
 // (problem_postprocessor/swe_post_wet_dry.hpp:5-145).  qv[var][vertex] in/out; returns the
-// new wet flag; *rewrite says whether the modal state must be rebuilt from qv.
+// new wet flag; *rewrite says whether the modal state must be rebuilt from qv.  tol widens the
+// comparisons with h_o (dry vertex, flattening, re-wetting) by the rounding of the stored state:
+// 0 for FP64 states, which reproduce the reference exactly.
 RKDG_HD inline bool wet_dry_vertices(double qv[3][3], const double* bv, double bmin, bool wet, double h_o,
-                                     double thr, bool* rewrite) {
+                                     double thr, double tol, bool* rewrite) {
     *rewrite = false;
     double h[3];
     unsigned n_dry = 0;
     for (int v = 0; v < 3; ++v) {
         h[v] = qv[0][v] + bv[v];
-        if (h[v] <= h_o + thr) ++n_dry;
+        if (h[v] <= h_o + thr + tol) ++n_dry;
     }
     double h_avg = (((0.0 + h[0]) + h[1]) + h[2]) / 3;
     if (h_avg <= thr) h_avg = thr;
@@ -426,7 +443,7 @@ RKDG_HD inline bool wet_dry_vertices(double qv[3][3], const double* bv, double b
         *rewrite = true;
         return false;
     } else {
-        if (h_avg <= h_o + thr) {
+        if (h_avg <= h_o + thr + tol) {
             for (int v = 0; v < 3; ++v) qv[0][v] = h_avg - bv[v], qv[1][v] = 0, qv[2][v] = 0;
         } else {
             int hmin = 0, hmax = 0;
@@ -443,14 +460,14 @@ RKDG_HD inline bool wet_dry_vertices(double qv[3][3], const double* bv, double b
             double dqx = 0, dqy = 0;
             unsigned nd = 0;
             for (int v = 0; v < 3; ++v)
-                if (h[v] <= h_o + thr) {
+                if (h[v] <= h_o + thr + tol) {
                     nd++;
                     dqx += qv[1][v], dqy += qv[2][v];
                     qv[1][v] = 0.0, qv[2][v] = 0.0;
                 }
             for (int v = 0; v < 3; ++v) {
                 qv[0][v] = h[v] - bv[v];
-                if (h[v] > h_o + thr) {
+                if (h[v] > h_o + thr + tol) {
                     qv[1][v] += dqx / (3 - nd);
                     qv[2][v] += dqy / (3 - nd);
                 }
@@ -464,7 +481,7 @@ RKDG_HD inline bool wet_dry_vertices(double qv[3][3], const double* bv, double b
         int hmax = 0;
         for (int v = 1; v < 3; ++v)
             if (h[hmax] < h[v]) hmax = v;
-        if (qv[0][hmax] > h_o - bmin) {
+        if (qv[0][hmax] > h_o - bmin + tol) {
             new_wet = true;
         } else {
             new_wet = false;
@@ -529,8 +546,17 @@ RKDG_HD RKDG_FORCE_INLINE inline void finish_element(const StageArgs<Real>& A, i
         }
         const double bv[3] = {ldc(A.bath_vrtx, ne, 0, e), ldc(A.bath_vrtx, ne, 1, e), ldc(A.bath_vrtx, ne, 2, e)};
         const double bmin = rmin(rmin(bv[0], bv[1]), bv[2]);
+        // FP32 state: the vertex elevations come back from FP32 modes with a rounding of about
+        // 2^-23 of their size (1e-7 for elevations of metres), far above the reference's depth
+        // tolerance h_thr (1e-12 of a metre).  An element flattened at h_o would otherwise read as
+        // partly wet on one side of h_o, be taken as wet, and keep the momentum its wet
+        // neighbours' face fluxes gave it while dry (tens of metres per second in a centimetre
+        // of water): the comparisons with h_o allow 8 units of that rounding.
+        double tol = 0.0;
+        if constexpr (sizeof(Real) == 4)
+            tol = 0x1p-20 * rmax(rmax(fabs(qv[0][0]), fabs(qv[0][1])), fabs(qv[0][2]));
         bool rewrite = false;
-        const bool wet_new = wet_dry_vertices(qv, bv, bmin, wet, A.h_o, A.h_thr, &rewrite);
+        const bool wet_new = wet_dry_vertices(qv, bv, bmin, wet, A.h_o, A.h_thr, tol, &rewrite);
         if (rewrite)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
@@ -952,19 +978,13 @@ RKDG_HD inline double cbrt_rcpR(double x) {  // x^(-1/3): y <- y (4 - x y^3) / 3
     return y * (4.0 - x * (y * y * y)) * (1.0 / 3.0);
 }
 
-// Depth under the wave speed's square root.  FP32: a depth the FP64 wetting-drying pass left at
-// or just above zero at a vertex can come back from the FP32 modes a rounding error below zero at
-// a trace point; it is taken as zero there (the square root would give NaN).  FP64: unchanged.
-RKDG_HD inline float depth_for_speed(float h) { return h > 0.0f ? h : 0.0f; }
-RKDG_HD inline double depth_for_speed(double h) { return h; }
-
 // SPH: the spherical correction sp enters as in llf (wave speed sqrt(g h ((nx sp)^2 + ny^2)), the
 // in-side flux's x-terms scaled); otherwise sp = 1 and nx^2 + ny^2 = 1.
 template <typename Real, bool SPH = false>
 RKDG_HD inline void llf_fast(Real grav, const Real* qi, const Real* qx, Real bath, Real nx, Real ny, Real* F,
                              Real sp = Real(1)) {
     const Real h_in = qi[0] + bath, h_ex = qx[0] + bath;
-    const Real r_in = rcpR(h_in), r_ex = rcpR(h_ex);
+    const Real r_in = rcpR(depth_for_div(h_in)), r_ex = rcpR(depth_for_div(h_ex));
     const Real u_in = qi[1] * r_in, v_in = qi[2] * r_in;
     const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
     const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
@@ -988,7 +1008,7 @@ template <typename Real, bool SPH = false>
 RKDG_HD inline void llf_rel(Real grav, const Real* qi, const Real* qx, Real bath, Real pi, Real px, Real nx,
                             Real ny, Real* F, Real sp = Real(1)) {
     const Real h_in = qi[0] + bath, h_ex = qx[0] + bath;
-    const Real r_in = rcpR(h_in), r_ex = rcpR(h_ex);
+    const Real r_in = rcpR(depth_for_div(h_in)), r_ex = rcpR(depth_for_div(h_ex));
     const Real u_in = qi[1] * r_in, v_in = qi[2] * r_in;
     const Real u_ex = qx[1] * r_ex, v_ex = qx[2] * r_ex;
     const Real un_in = u_in * nx + v_in * ny, un_ex = u_ex * nx + v_ex * ny;
