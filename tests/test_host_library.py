@@ -2,6 +2,7 @@
 host-only helpers behave (no device work here)."""
 import ctypes as C
 import re
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -34,6 +35,22 @@ def test_create_rejects_bad_arguments():
     lib = N.load_library()
     h = C.c_void_p()
     assert lib.rkdg_create(None, None, 0, C.byref(h)) == N.RKDG_ERR_ARG
+
+
+def test_c_program_uses_the_abi(tmp_path):
+    """include/rkdg.h compiles as strict C11 (-pedantic -Werror) and a C program links librkdg.so and
+    drives it (tests/c_abi/lake_at_rest.c); without a HIP device it reports RKDG_ERR_NO_DEVICE
+    through rkdg_create and exits with status 2."""
+    import subprocess
+    import torch  # noqa: F401  (device count only, no HIP initialisation)
+    sys.path.insert(0, str(Path(__file__).resolve().parents[1]))
+    import __graft_entry__ as G
+    exe = G.build_c_abi_example(tmp_path / "lake_at_rest")
+    if torch.cuda.device_count() > 0:
+        pytest.skip("a HIP device is visible (the GPU test runs the program)")
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2, (r.returncode, r.stdout, r.stderr)
+    assert "no HIP device" in r.stdout
 
 
 def test_no_device_is_reported(monkeypatch):
