@@ -200,6 +200,71 @@ struct Engine final : EngineBase {
         for (hipEvent_t ev : ev_pool) hipEventDestroy(ev);
         for (void* p : allocs) hipFree(p);
         if (ramp_tab_h) hipHostFree(ramp_tab_h);
+        for (int i = 0; i < 2; ++i) {
+            if (pin[i]) hipHostFree(pin[i]);
+            if (pin_ev[i]) hipEventDestroy(pin_ev[i]);
+        }
+    }
+
+    // State transfers between the caller's FP64 host arrays and a device state slot go through two
+    // pinned staging buffers (allocated on first use): the device copy of one chunk runs while the
+    // previous chunk is converted into (or out of) the caller's array, instead of a pageable copy
+    // of the whole state into a freshly allocated host vector.
+    static constexpr size_t kPinBytes = size_t(8) << 20;
+    void* pin[2] = {nullptr, nullptr};
+    hipEvent_t pin_ev[2] = {nullptr, nullptr};
+    hipError_t ensure_pinned() {
+        for (int i = 0; i < 2; ++i) {
+            if (!pin[i]) {
+                const hipError_t e = hipHostMalloc(&pin[i], kPinBytes, hipHostMallocDefault);
+                if (e != hipSuccess) return pin[i] = nullptr, e;
+            }
+            if (!pin_ev[i]) {
+                const hipError_t e = hipEventCreateWithFlags(&pin_ev[i], hipEventDisableTiming);
+                if (e != hipSuccess) return pin_ev[i] = nullptr, e;
+            }
+        }
+        return hipSuccess;
+    }
+    // device -> host: chunk c is copied into buffer c % 2 while chunk c - 1 is converted; buffer
+    // c % 2 was last read when chunk c - 2 was converted, before chunk c's copy is issued
+    hipError_t read_state(const Real* dev, double* out, size_t n) {
+        hipError_t e = ensure_pinned();
+        if (e != hipSuccess) return e;
+        const size_t per = kPinBytes / sizeof(Real), nch = (n + per - 1) / per;
+        for (size_t c = 0; c <= nch && e == hipSuccess; ++c) {
+            if (c < nch) {
+                const size_t off = c * per, cnt = std::min(per, n - off);
+                e = hipMemcpyAsync(pin[c & 1], dev + off, cnt * sizeof(Real), hipMemcpyDeviceToHost, stream);
+                if (e == hipSuccess) e = hipEventRecord(pin_ev[c & 1], stream);
+            }
+            if (e == hipSuccess && c > 0) {
+                const size_t pc = c - 1, off = pc * per, cnt = std::min(per, n - off);
+                e = hipEventSynchronize(pin_ev[pc & 1]);
+                const Real* src = static_cast<const Real*>(pin[pc & 1]);
+                if (e == hipSuccess)
+                    for (size_t i = 0; i < cnt; ++i) out[off + i] = (double)src[i];
+            }
+        }
+        return e;
+    }
+    // host -> device: chunk c is converted into buffer c % 2 once that buffer's previous copy
+    // (chunk c - 2) has completed, then copied while chunk c + 1 is converted
+    hipError_t write_state(Real* dev, const double* in, size_t n) {
+        hipError_t e = ensure_pinned();
+        if (e != hipSuccess) return e;
+        const size_t per = kPinBytes / sizeof(Real), nch = (n + per - 1) / per;
+        for (size_t c = 0; c < nch && e == hipSuccess; ++c) {
+            const size_t off = c * per, cnt = std::min(per, n - off);
+            if (c >= 2) e = hipEventSynchronize(pin_ev[c & 1]);
+            Real* dst = static_cast<Real*>(pin[c & 1]);
+            if (e == hipSuccess) {
+                for (size_t i = 0; i < cnt; ++i) dst[i] = (Real)in[off + i];
+                e = hipMemcpyAsync(dev + off, dst, cnt * sizeof(Real), hipMemcpyHostToDevice, stream);
+            }
+            if (e == hipSuccess) e = hipEventRecord(pin_ev[c & 1], stream);
+        }
+        return e;
     }
 
     bool fail(const char* what, hipError_t e) {
@@ -676,8 +741,7 @@ struct Engine final : EngineBase {
     rkdg_status set_state(const double* qh, const uint8_t* wh) override {
         const size_t S = 3 * (size_t)M.ndof * M.ne;
         if (qh) {
-            std::vector<Real> tmp(qh, qh + S);
-            hipError_t e = hipMemcpyAsync(q[M.stepper.stage], tmp.data(), S * sizeof(Real), hipMemcpyHostToDevice, stream);
+            hipError_t e = write_state(q[M.stepper.stage], qh, S);
             // the new state replaces the one a recorded NaN was found in: clear the record
             if (e == hipSuccess) e = hipMemcpyAsync(d_nan, &kNoNan, sizeof kNoNan, hipMemcpyHostToDevice, stream);
             if (e == hipSuccess) e = hipStreamSynchronize(stream);
@@ -696,10 +760,8 @@ struct Engine final : EngineBase {
         hipError_t e = hipStreamSynchronize(stream);
         if (e != hipSuccess) return fail("synchronize", e), RKDG_ERR_HIP;
         if (qh) {
-            std::vector<Real> tmp(S);
-            e = hipMemcpy(tmp.data(), q[M.stepper.stage], S * sizeof(Real), hipMemcpyDeviceToHost);
+            e = read_state(q[M.stepper.stage], qh, S);
             if (e != hipSuccess) return fail("get_state", e), RKDG_ERR_HIP;
-            for (size_t i = 0; i < S; ++i) qh[i] = (double)tmp[i];
         }
         if (wh) {
             e = hipMemcpy(wh, wet[wcur], ne, hipMemcpyDeviceToHost);
