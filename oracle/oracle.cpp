@@ -13,7 +13,8 @@
 // (test/test_basis_dubiner_2d.cpp), quadrature exactness (test_integ_*), LLF golden vectors
 // (test_llf_flux.cpp), element integration golden vectors (test_element_triangle.hpp),
 // the five wetting-drying scenarios (test_wetting_drying.cpp), the RK ODE test
-// (test_rk_stepper.cpp), plus invariants (lake at rest, mass conservation, manufactured-
+// (test_rk_stepper.cpp), the Runge-Kutta order conditions of the SSP-RK tables it is handed
+// for the schemes of five to eight stages (tests/test_oracle_unit.py), plus invariants (lake at rest, mass conservation, manufactured-
 // solution convergence, limiter invariants), closed forms of one forward-Euler stage for
 // bottom friction and tide / flow forcing (tests/test_oracle_forcing.py), of the trouble
 // check on a step (tests/test_oracle_trouble.py), and of the Cockburn-Shu limiting step
@@ -493,6 +494,27 @@ struct Stepper {
     std::vector<std::vector<double>> ark, brk, crk;
     std::vector<double> drk;
 
+    // Schemes other than the four restated below (explicit_ssp_rk_stepper.cpp:93-257: five to
+    // eight stages, orders 3 and 4, whose published coefficients are decimal tables) take their
+    // Shu-Osher tables from the caller (orc_register_scheme, row-major ns x ns, entries above the
+    // diagonal ignored).  The tests hand in tables they have first checked against the Runge-Kutta
+    // order conditions and the SSP sign conditions.
+    bool registered(unsigned ns, unsigned ord) {
+        auto it = scheme_registry().find({ns, ord});
+        if (it == scheme_registry().end()) return false;
+        for (unsigned i = 0; i < ns; ++i)
+            for (unsigned k = 0; k <= i; ++k) {
+                ark[i][k] = it->second.first[i * ns + k];
+                brk[i][k] = it->second.second[i * ns + k];
+            }
+        return true;
+    }
+    static std::map<std::pair<unsigned, unsigned>, std::pair<std::vector<double>, std::vector<double>>>&
+    scheme_registry() {
+        static std::map<std::pair<unsigned, unsigned>, std::pair<std::vector<double>, std::vector<double>>> r;
+        return r;
+    }
+
     bool init(unsigned ns, unsigned ord, double dt_, double ramp_dur) {
         nstages = ns, order = ord, dt = dt_, ramp_duration = ramp_dur;
         ramp = (std::abs(ramp_duration) < std::numeric_limits<double>::epsilon() * 100) ? 1. : 0.;
@@ -515,8 +537,8 @@ struct Stepper {
         } else if (ns == 4 && ord == 3) {
             a[0][0] = 1., a[1][1] = 1., a[2][0] = 2. / 3., a[2][2] = 1. / 3., a[3][3] = 1.;
             b[0][0] = 1. / 2., b[1][1] = 1. / 2., b[2][2] = 1. / 6., b[3][3] = 1. / 2.;
-        } else {
-            return false;  // other schemes: not restated here (the engine's own table covers them)
+        } else if (!registered(ns, ord)) {
+            return false;
         }
         // time-dependent parameters (.cpp:259-276)
         for (unsigned i = 0; i < ns; ++i)
@@ -1869,6 +1891,15 @@ int orc_rk_coeffs(int nstages, int order, double* ark, double* brk, double* drk)
             brk[i * nstages + k] = k <= i ? st.brk[i][k] : 0.0;
         }
     for (int i = 0; i < nstages; ++i) drk[i] = st.drk[i];
+    return 0;
+}
+
+// Shu-Osher tables of a scheme the oracle does not restate, for contexts created afterwards.
+int orc_register_scheme(int nstages, int order, const double* ark, const double* brk) {
+    if (nstages < 1 || nstages > 8 || order < 1 || !ark || !brk) return 1;
+    const size_t n = (size_t)nstages * nstages;
+    orc::Stepper::scheme_registry()[{(unsigned)nstages, (unsigned)order}] = {std::vector<double>(ark, ark + n),
+                                                                             std::vector<double>(brk, brk + n)};
     return 0;
 }
 

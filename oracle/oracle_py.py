@@ -47,6 +47,7 @@ def load():
             "orc_llf_flux": (None, [C.c_double, dp, dp, dp, dp, dp]),
             "orc_rk_coeffs": (C.c_int, [C.c_int, C.c_int, dp, dp, dp]),
             "orc_stage_times": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_int, dp]),
+            "orc_register_scheme": (C.c_int, [C.c_int, C.c_int, dp, dp]),
             "orc_element_integrals": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp]),
             "orc_element_gp_global": (C.c_int, [C.c_int, dp, dp, dp]),
             "orc_element_edge_integrals": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp]),
@@ -77,6 +78,14 @@ class OracleSim:
         if not self.h:
             raise RuntimeError("oracle create failed: " + self.lib.orc_error().decode())
         self.lib.orc_set_threads(self.h, threads)
+
+    @staticmethod
+    def register_scheme(nstages, order, ark, brk):
+        """Shu-Osher tables ([nstages, nstages], lower triangle) of a scheme the oracle does not
+        restate, for oracles created afterwards."""
+        a = np.ascontiguousarray(ark, np.float64)
+        b = np.ascontiguousarray(brk, np.float64)
+        return load().orc_register_scheme(nstages, order, N.dptr(a), N.dptr(b))
 
     def close(self):
         if self.h:

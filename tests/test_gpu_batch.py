@@ -61,9 +61,10 @@ def test_tidal_batch_equals_single_runs(precision):
         b.close()
 
 
-@pytest.mark.parametrize("nstages,order", [(2, 2), (3, 3)])
+@pytest.mark.parametrize("nstages,order", [(2, 2), (3, 3), (5, 4), (8, 4)])
 def test_wetting_drying_limiter_batch_equals_single_runs(nstages, order):
-    # dam break with wetting-drying and the limiter; 3 stages exercise the odd-stage wet-flag copy
+    # dam break with wetting-drying and the limiter; 3 and 5 stages exercise the odd-stage wet-flag
+    # copy, 8 stages the per-instance coefficient table at its full eight terms
     cases = []
     for scale in (0.04, 0.06, 0.05):
         mesh, prm = W.dambreak(scale=scale)

@@ -21,7 +21,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import relerr
+from conftest import SSPRK_SCHEMES, relerr, rk_scheme_ok
 from dgswemv2_amd import (BC_LAND, PROBLEM_DAM_BREAK, RKDG_FP32_BULK, RKDG_FP64_FAST, RKDG_FP64_PLAIN, RkdgError, Simulation,
                           make_params, quad_mesh, workloads as W)
 from dgswemv2_amd import _native as N
@@ -61,6 +61,27 @@ def compare_run(name, nsteps, per_stage=False, threads=1, **kw):
     assert g.time()[:2] == o.time()[:2]
     banner(name, "worst rel L2", worst, "bit-exact", exact)
     return worst, exact
+
+
+def register_ssprk_schemes():
+    """Hand the oracle the Shu-Osher tables of the schemes it does not restate (five to eight
+    stages), after checking them against the Runge-Kutta order and SSP sign conditions."""
+    lib = N.load_library()
+    for ns, order in SSPRK_SCHEMES:
+        if ns <= 4 and order <= 3:
+            continue
+        a, b, d = np.zeros((ns, ns)), np.zeros((ns, ns)), np.zeros(ns)
+        assert lib.rkdg_ssprk_coeffs(ns, order, N.dptr(a), N.dptr(b), N.dptr(d)) == 0
+        assert rk_scheme_ok(a, b, order), (ns, order)
+        assert OracleSim.register_scheme(ns, order, a, b) == 0
+
+
+@pytest.mark.parametrize("ns,order", SSPRK_SCHEMES)
+def test_every_ssprk_scheme_per_stage(ns, order):
+    # The reference stepper's twelve schemes (explicit_ssp_rk_stepper.cpp:16-257): a stage of the
+    # eight-stage schemes combines up to eight earlier states and stage solutions.
+    register_ssprk_schemes()
+    compare_run("tidal", 3, per_stage=True, scale=0.03, nstages=ns, order=order)
 
 
 def test_manufactured_p1_rk2():
@@ -183,6 +204,7 @@ def test_dambreak_p1():
     # p = 3: the fast form's per-step rounding against the reference order grows with the order
     # (this dam break: worst 1e-13 at p = 1, 3e-13 at p = 2, 1.1e-12 at p = 3, no flag differing)
     ("dambreak", dict(scale=0.05, p=3), 20, 0.0, 3e-12),
+    ("tidal", dict(scale=0.05, nstages=8, order=4), 6, 0.0, TOL64),
 ])
 def test_fp64_fast_against_oracle(name, kw, steps, max_flip_frac, tol):
     # The per-step L2 bound below is taken after the limiter, so it covers the effect of any
@@ -190,6 +212,7 @@ def test_fp64_fast_against_oracle(name, kw, steps, max_flip_frac, tol):
     # flipped write-back moves the linear modes by about 1e-12 of their size; at p >= 2 it also
     # zeroes (or keeps) the element's higher modes, so the workloads above that flip decisions
     # (the bowl) are run at p = 1 here.
+    register_ssprk_schemes()
     mesh, prm = W.WORKLOADS[name](**kw)
     banner(name, kw, "fp64-fast ne", mesh.n_elements, "steps", steps)
     o = OracleSim(mesh, prm)

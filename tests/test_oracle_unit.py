@@ -14,6 +14,7 @@ from pathlib import Path
 import numpy as np
 import pytest
 
+from conftest import SSPRK_SCHEMES, rk_scheme_ok
 from oracle.oracle_py import load
 
 GOLD = Path(__file__).resolve().parent / "golden"
@@ -178,6 +179,38 @@ def test_engine_stepper_tables(ns, order):
     if ns <= 4:
         ao, bo, do = rk_coeffs_oracle(ns, order)
         assert np.array_equal(a, ao) and np.array_equal(b, bo) and np.array_equal(dd, do)
+
+
+@pytest.mark.parametrize("ns,order", SSPRK_SCHEMES)
+def test_engine_stepper_order_conditions(ns, order):
+    # The engine's Shu-Osher tables (from the reference's published decimal coefficients) satisfy
+    # the Runge-Kutta order conditions of their stated order within 1e-9 (the five-stage
+    # fourth-order table, published to 11-12 digits, within 9e-11) and the SSP sign conditions;
+    # a one-entry change of 1e-6 in any nonzero coefficient breaks them.
+    a, b, _ = rk_coeffs_engine(ns, order)
+    assert rk_scheme_ok(a, b, order)
+    if order < 4:  # not of the next order (the tables are not accidentally better)
+        assert not rk_scheme_ok(a, b, order + 1)
+    for i, k in zip(*np.nonzero(np.tril(a) + np.tril(b))):
+        for m in (a, b):
+            if m[i, k] != 0.0:
+                m[i, k] += 1e-6
+                assert not rk_scheme_ok(a, b, order), (i, k)
+                m[i, k] -= 1e-6
+
+
+@pytest.mark.parametrize("ns,order", SSPRK_SCHEMES)
+def test_oracle_registered_scheme_matches_engine(ns, order):
+    # The oracle restates four schemes; the others are handed in as tables.  Its stage-time
+    # offsets drk (the crk recursion, explicit_ssp_rk_stepper.cpp:259-276) must equal the
+    # engine's for every scheme, and a registered table reads back unchanged.
+    lib = load()
+    a, b, dd = rk_coeffs_engine(ns, order)
+    if (ns, order) not in ((1, 1), (2, 2), (3, 3), (4, 3)):
+        assert lib.orc_register_scheme(ns, order, a.ctypes.data_as(dp), b.ctypes.data_as(dp)) == 0
+    ao, bo, do = rk_coeffs_oracle(ns, order)
+    assert np.array_equal(np.tril(a), ao) and np.array_equal(np.tril(b), bo)
+    assert np.array_equal(dd, do)
 
 
 @pytest.mark.parametrize("ns,order", [(2, 2), (3, 3)])
