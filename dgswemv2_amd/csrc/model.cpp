@@ -15,6 +15,7 @@ namespace rkdg {
 
 // ---------------------------------------------------------------------------------------
 bool Stepper::init(int ns, int ord, double dt_, double ramp_days) {
+    if (ns < 1 || ns > 8 || ord < 1) return false;  // the published schemes have 1-8 stages
     nstages = ns, order = ord, dt = dt_, ramp_duration = ramp_days;
     step = 0, stage = 0, t = 0;
     const bool no_ramp = std::abs(ramp_duration) < std::numeric_limits<double>::epsilon() * 100;

@@ -102,8 +102,8 @@ typedef struct rkdg_mesh_desc {
  * problem_preprocessor/swe_pre_init_prob_params.hpp:5-47). */
 typedef struct rkdg_params {
     int32_t p;               /* polynomial order: 1, 2 or 3 */
-    int32_t nstages;         /* SSP-RK stages */
-    int32_t order;           /* SSP-RK order */
+    int32_t nstages;         /* SSP-RK stages; (nstages, order) is one of the reference's twelve   */
+    int32_t order;           /* schemes: (1,1) (2,2) (3,3) (4..8,3) (5..8,4); others: RKDG_ERR_ARG */
     double dt;
     double g;
     double ramp_duration;    /* days; 0 = no ramp */

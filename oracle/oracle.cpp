@@ -516,6 +516,7 @@ struct Stepper {
     }
 
     bool init(unsigned ns, unsigned ord, double dt_, double ramp_dur) {
+        if (ns < 1 || ns > 8) return false;
         nstages = ns, order = ord, dt = dt_, ramp_duration = ramp_dur;
         ramp = (std::abs(ramp_duration) < std::numeric_limits<double>::epsilon() * 100) ? 1. : 0.;
         ark.assign(ns, {}), brk.assign(ns, {}), crk.assign(ns, {});

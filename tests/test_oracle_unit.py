@@ -199,6 +199,14 @@ def test_engine_stepper_order_conditions(ns, order):
                 m[i, k] -= 1e-6
 
 
+@pytest.mark.parametrize("ns,order", [(0, 1), (1, 2), (2, 3), (4, 4), (9, 3), (3, 5), (-1, 3)])
+def test_engine_rejects_schemes_outside_the_twelve(ns, order):
+    from dgswemv2_amd import _native as N
+    lib = N.load_library()
+    a, b, dd = np.zeros(100), np.zeros(100), np.zeros(10)
+    assert lib.rkdg_ssprk_coeffs(ns, order, N.dptr(a), N.dptr(b), N.dptr(dd)) == N.RKDG_ERR_ARG
+
+
 @pytest.mark.parametrize("ns,order", SSPRK_SCHEMES)
 def test_oracle_registered_scheme_matches_engine(ns, order):
     # The oracle restates four schemes; the others are handed in as tables.  Its stage-time
