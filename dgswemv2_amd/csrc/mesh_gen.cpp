@@ -44,10 +44,17 @@ struct Fourier {
     double mean = 0, lo = 0, hi = 0;
 };
 
+constexpr int kMaxModes = 4096;  // Fourier bathymetry modes
+
 }  // namespace
 
 extern "C" rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* sp, rkdg_mesh** out, rkdg_mesh_desc* desc) {
     if (!sp || !out || !desc || sp->m < 1 || sp->n < 1 || sp->pattern < 0 || sp->pattern > 2) return RKDG_ERR_ARG;
+    if (!(sp->x2 > sp->x1) || !(sp->y2 > sp->y1) || sp->bathy_kind < 0 || sp->bathy_kind > 3 ||
+        sp->n_modes > kMaxModes)
+        return RKDG_ERR_ARG;
+    for (int s = 0; s < 4; ++s)
+        if (sp->bc[s] < RKDG_BC_LAND || sp->bc[s] > RKDG_BC_FUNCTION) return RKDG_ERR_ARG;
     return rkdg::guarded("rkdg_mesh_generate", [&]() -> rkdg_status {
     std::unique_ptr<rkdg_mesh> owner(new rkdg_mesh());
     rkdg_mesh* M = owner.get();

@@ -236,7 +236,7 @@ typedef struct rkdg_quad_mesh_spec {
     int32_t bathy_kind;         /* 0 constant(b0), 1 bowl -alpha*(x^2+y^2), 2 Fourier modes, 3 step */
     double b0, alpha;           /* constant depth / bowl coefficient */
     double bmin, bmax;          /* Fourier clamp range */
-    int32_t n_modes;            /* Fourier modes */
+    int32_t n_modes;            /* Fourier modes (<= 0: 16; at most 4096) */
     double jitter;              /* interior vertex jitter as a fraction of spacing */
     uint64_t seed;
     double step_x, step_b_left, step_b_right; /* bathy_kind 3: depth b_left for x<step_x else b_right */
@@ -246,6 +246,8 @@ typedef struct rkdg_quad_mesh_spec {
 } rkdg_quad_mesh_spec;
 
 typedef struct rkdg_mesh rkdg_mesh; /* owns the arrays a rkdg_mesh_desc points into */
+/* RKDG_ERR_ARG for m or n < 1, an empty rectangle, an unknown pattern, bathymetry kind or side
+ * type (land, tide, flow, function), or more than 4096 Fourier modes. */
 rkdg_status rkdg_mesh_generate(const rkdg_quad_mesh_spec* spec, rkdg_mesh** out, rkdg_mesh_desc* desc);
 
 /*

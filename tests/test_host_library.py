@@ -77,6 +77,39 @@ def test_mesh_generator_patterns():
         assert len(m.bound_types) == 2 * (3 + 2)
 
 
+def _generate(**fields):
+    lib = N.load_library()
+    s = N.QuadMeshSpec()
+    s.m, s.n, s.x2, s.y2, s.b0 = 4, 3, 1.0, 1.0, 2.0
+    for k, v in fields.items():
+        if k == "bc":
+            for i, b in enumerate(v):
+                s.bc[i] = b
+        else:
+            setattr(s, k, v)
+    handle, desc = C.c_void_p(), N.MeshDesc()
+    st = lib.rkdg_mesh_generate(C.byref(s), C.byref(handle), C.byref(desc))
+    if st == N.RKDG_OK:
+        lib.rkdg_mesh_free(handle)
+    return st
+
+
+@pytest.mark.parametrize("fields", [dict(m=0), dict(n=-2), dict(m=1 << 30, n=1 << 30), dict(pattern=3),
+                                    dict(x2=0.0), dict(y1=2.0), dict(bathy_kind=4), dict(bathy_kind=-1),
+                                    dict(bathy_kind=2, bmin=5.0, bmax=10.0, n_modes=1 << 30), dict(bc=[0, 9, 0, 0]),
+                                    dict(bc=[0, 0, 0, 129])])
+def test_mesh_generator_rejects_bad_specs(fields):
+    # every field the generator sizes or branches on is checked before any work (a huge mode
+    # count would otherwise run for hours, an unknown side type reach the model builder)
+    assert _generate(**fields) == N.RKDG_ERR_ARG
+
+
+def test_mesh_generator_accepts_the_documented_ranges():
+    assert _generate() == N.RKDG_OK
+    assert _generate(bathy_kind=2, bmin=5.0, bmax=10.0, n_modes=4096) == N.RKDG_OK
+    assert _generate(bc=[1, 2, 3, 0], frequency=1e-4, forcing_fact=1.0, amplitude=0.1) == N.RKDG_OK
+
+
 def test_noalloc_forward_rejects_bad_arguments_before_any_device_work():
     lib = N.load_library()
     w = N.MlpWeights()
