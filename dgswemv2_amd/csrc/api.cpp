@@ -881,11 +881,6 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
         g_create_error = "unknown precision";
         return RKDG_ERR_ARG;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        g_create_error = "no HIP device visible";
-        return RKDG_ERR_NO_DEVICE;
-    }
     std::unique_ptr<rkdg_ctx> c(new (std::nothrow) rkdg_ctx());
     if (!c) return RKDG_ERR_ARG;
     c->precision = precision;
@@ -898,6 +893,12 @@ rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, r
     if (!msg.empty()) {
         g_create_error = msg;
         return params->p >= 0 && !rkdg::order_built(params->p) ? RKDG_ERR_UNSUPPORTED : RKDG_ERR_ARG;
+    }
+    // the mesh and parameters are checked first, so argument errors do not depend on a device
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device visible";
+        return RKDG_ERR_NO_DEVICE;
     }
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
@@ -1246,11 +1247,6 @@ rkdg_status rkdg_create_batch(int32_t n, const rkdg_mesh_desc* meshes, const rkd
         g_create_error = "unknown precision";
         return RKDG_ERR_ARG;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        g_create_error = "no HIP device visible";
-        return RKDG_ERR_NO_DEVICE;
-    }
     std::unique_ptr<rkdg_ctx> c(new (std::nothrow) rkdg_ctx());
     if (!c) return RKDG_ERR_ARG;
     c->precision = precision;
@@ -1293,6 +1289,11 @@ rkdg_status rkdg_create_batch(int32_t n, const rkdg_mesh_desc* meshes, const rkd
             g_create_error = "all instances need the same number of Runge-Kutta stages";
             return RKDG_ERR_ARG;
         }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        g_create_error = "no HIP device visible";
+        return RKDG_ERR_NO_DEVICE;
+    }
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         g_create_error = "hipStreamCreate failed";
         return RKDG_ERR_HIP;

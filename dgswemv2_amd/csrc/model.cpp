@@ -121,6 +121,15 @@ std::vector<double> shape_psi(const std::vector<std::array<double, 2>>& pts) {
 std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model& M, bool reference_face_order) {
     M.reference_face_order = reference_face_order;
     if (md.n_elements <= 0 || md.n_nodes <= 0 || !md.node_xyz || !md.elt_nodes) return "empty mesh";
+    if (md.n_bound_edges < 0 || (md.n_bound_edges > 0 && (!md.bound_edge_nodes || !md.bound_edge_type)))
+        return "boundary edge count negative or its arrays missing";
+    if (md.n_constituents < 0) return "negative constituent count";
+    if (md.n_constituents > 0 && (!md.frequency || !md.forcing_fact || !md.equilib_arg))
+        return "constituent arrays missing";
+    if (md.n_levee_pairs < 0 || (md.n_levee_pairs > 0 && (!md.levee_nodes || !md.levee_param)))
+        return "levee pair count negative or its arrays missing";
+    if (prm.problem < RKDG_PROBLEM_NONE || prm.problem > RKDG_PROBLEM_DAM_BREAK) return "unknown problem";
+    if (prm.friction < RKDG_FRICTION_NONE || prm.friction > RKDG_FRICTION_MANNING) return "unknown friction law";
     if (prm.p < 1 || prm.p > 3) return "polynomial order outside the device-built set {1, 2, 3}";
     // the kernels address every element-fastest array with 32-bit byte offsets
     if ((size_t)md.n_elements * 3 * 6 * sizeof(double) >= ((size_t)1 << 32))
@@ -142,6 +151,14 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
             const int v = md.elt_nodes[3 * e + k];
             if (v < 0 || v >= md.n_nodes) return "element node index out of range";
         }
+    for (size_t i = 0; i < 3 * (size_t)md.n_nodes; ++i)
+        if (!std::isfinite(md.node_xyz[i])) return "non-finite node coordinate or depth";
+    for (int e = 0; e < ne; ++e) {  // collinear vertices (a repeated node included): no Jacobian inverse
+        const double* a = md.node_xyz + 3 * (size_t)md.elt_nodes[3 * e];
+        const double* b = md.node_xyz + 3 * (size_t)md.elt_nodes[3 * e + 1];
+        const double* c = md.node_xyz + 3 * (size_t)md.elt_nodes[3 * e + 2];
+        if ((b[0] - a[0]) * (c[1] - a[1]) - (c[0] - a[0]) * (b[1] - a[1]) == 0.0) return "degenerate element (zero area)";
+    }
 
     // ---- edges by node pair (mesh_metadata.cpp:31-85)
     std::map<std::pair<int, int>, uint8_t> btype_of;
@@ -304,7 +321,6 @@ std::string build_model(const rkdg_mesh_desc& md, const rkdg_params& prm, Model&
             for (int k = 0; k < 3; ++k)
                 if (btype[3 * e + k] == ty) bedges.push_back({ty, e, k});
     M.ncon = md.n_constituents;
-    if (M.ncon < 0) return "negative constituent count";
     for (int c = 0; c < M.ncon; ++c) {
         M.freq.push_back(md.frequency[c]);
         M.ffact.push_back(md.forcing_fact[c]);

@@ -154,7 +154,11 @@ typedef struct rkdg_perf_report {
  *   problem_preprocessor/swe_pre_init_data.hpp:8-327) and the ESSPRKStepper ctor
  *   (simulation/stepper/explicit_ssp_rk_stepper.cpp:3-14).
  * RKDG_ERR_UNSUPPORTED for a polynomial order the kernels are not built for (p = 0 or p > 3),
- * RKDG_ERR_ARG for any other invalid description (message from rkdg_create_error). */
+ * RKDG_ERR_ARG for any other invalid description (message from rkdg_create_error): negative
+ * counts, positive counts with NULL arrays, node indices out of range, non-finite coordinates or
+ * depths, zero-area elements, edges of more than two elements or without a boundary type, unknown
+ * problem, friction law or SSP-RK scheme.  The description is checked before the device is looked
+ * for, so these errors come first on any machine; RKDG_ERR_NO_DEVICE after them. */
 rkdg_status rkdg_create(const rkdg_mesh_desc* mesh, const rkdg_params* params, rkdg_precision precision,
                         rkdg_ctx** out);
 void rkdg_destroy(rkdg_ctx* ctx);

@@ -37,6 +37,87 @@ def test_create_rejects_bad_arguments():
     assert lib.rkdg_create(None, None, 0, C.byref(h)) == N.RKDG_ERR_ARG
 
 
+def _set_array(desc, field, values, ctype, keep):
+    a = np.ascontiguousarray(values)
+    keep.append(a)
+    setattr(desc, field, a.ctypes.data_as(C.POINTER(ctype)))
+
+
+def _elements(d):
+    return np.ctypeslib.as_array(d.elt_nodes, (d.n_elements * 3,)).copy()
+
+
+def _nodes(d):
+    return np.ctypeslib.as_array(d.node_xyz, (d.n_nodes * 3,)).copy()
+
+
+def _degenerate(d, keep):
+    x, e = _nodes(d), _elements(d)
+    x[3 * e[1]:3 * e[1] + 2] = x[3 * e[0]:3 * e[0] + 2]  # vertex 1 of element 0 onto vertex 0
+    _set_array(d, "node_xyz", x, C.c_double, keep)
+
+
+def _node_oob(d, keep):
+    e = _elements(d)
+    e[7] = d.n_nodes
+    _set_array(d, "elt_nodes", e, C.c_int32, keep)
+
+
+def _nan_depth(d, keep):
+    x = _nodes(d)
+    x[5] = np.nan
+    _set_array(d, "node_xyz", x, C.c_double, keep)
+
+
+BAD_CREATE = {
+    # mesh description: counts against their arrays, indices, geometry
+    "negative elements": (lambda d, p, k: setattr(d, "n_elements", -5), "empty mesh"),
+    "node index out of range": (lambda d, p, k: _node_oob(d, k), "element node index out of range"),
+    "zero-area element": (lambda d, p, k: _degenerate(d, k), "degenerate element"),
+    "non-finite depth": (lambda d, p, k: _nan_depth(d, k), "non-finite node coordinate"),
+    "boundary edges without arrays": (lambda d, p, k: setattr(d, "bound_edge_nodes", None), "boundary edge count"),
+    "negative boundary edges": (lambda d, p, k: setattr(d, "n_bound_edges", -3), "boundary edge count"),
+    "constituents without arrays": (lambda d, p, k: setattr(d, "frequency", None), "constituent arrays missing"),
+    "negative constituents": (lambda d, p, k: setattr(d, "n_constituents", -2), "negative constituent count"),
+    "levee pairs without arrays": (lambda d, p, k: setattr(d, "n_levee_pairs", 2), "levee pair count"),
+    "negative levee pairs": (lambda d, p, k: setattr(d, "n_levee_pairs", -4), "levee pair count"),
+    "tide without amplitudes": (lambda d, p, k: setattr(d, "node_amplitude", None), "without forcing data"),
+    # run parameters
+    "unknown problem": (lambda d, p, k: setattr(p, "problem", 99), "unknown problem"),
+    "unknown friction law": (lambda d, p, k: setattr(p, "friction", 9), "unknown friction law"),
+    "negative stage count": (lambda d, p, k: setattr(p, "nstages", -3), "invalid Runge-Kutta method"),
+    "negative order": (lambda d, p, k: setattr(p, "p", -3), "polynomial order"),
+}
+
+
+@pytest.mark.parametrize("case", list(BAD_CREATE))
+def test_create_checks_mesh_and_parameters_before_the_device(case):
+    # every count is checked against its array and every index against its range before any
+    # array is read, and before the device is looked for, so a malformed description gets
+    # RKDG_ERR_ARG with a message on any machine (null arrays behind positive counts, for
+    # instance, read through a null pointer otherwise)
+    from dgswemv2_amd import workloads
+    lib = N.load_library()
+    mesh, prm = workloads.tidal(scale=0.02)
+    desc, keep = mesh.desc, []
+    assert desc.n_constituents > 0 and desc.n_bound_edges > 0
+    mutate, msg = BAD_CREATE[case]
+    mutate(desc, prm, keep)
+    h = C.c_void_p()
+    assert lib.rkdg_create(C.byref(desc), C.byref(prm), N.RKDG_FP64_PLAIN, C.byref(h)) == N.RKDG_ERR_ARG
+    assert msg in lib.rkdg_create_error().decode()
+    assert not h.value
+
+
+def test_unbuilt_order_is_unsupported_before_the_device():
+    from dgswemv2_amd import workloads
+    lib = N.load_library()
+    mesh, prm = workloads.tidal(scale=0.02)
+    prm.p = 4
+    h = C.c_void_p()
+    assert lib.rkdg_create(C.byref(mesh.desc), C.byref(prm), N.RKDG_FP64_PLAIN, C.byref(h)) == N.RKDG_ERR_UNSUPPORTED
+
+
 def test_c_program_uses_the_abi(tmp_path):
     """include/rkdg.h compiles as strict C11 (-pedantic -Werror) and a C program links librkdg.so and
     drives it (tests/c_abi/lake_at_rest.c); without a HIP device it reports RKDG_ERR_NO_DEVICE
