@@ -75,6 +75,15 @@ class Mesh:
     def n_elements(self):
         return len(self.elt_nodes)
 
+    def with_manning(self, node_manning_n):
+        """The same mesh with nodal Manning n (one value per node)."""
+        n = np.ascontiguousarray(node_manning_n, np.float64).reshape(-1)
+        if len(n) != len(self.node_xyz):
+            raise ValueError("one Manning n per node")
+        return Mesh(self.node_xyz, self.elt_nodes, self.bound_edges, self.bound_types, self.frequency,
+                    self.forcing_fact, self.equilib_arg, self.node_amplitude, self.node_phase, n,
+                    self.levee_nodes, self.levee_param)
+
     def min_edge_length(self):
         x, y = self.node_xyz[:, 0], self.node_xyz[:, 1]
         e = self.elt_nodes
