@@ -475,6 +475,9 @@ class Simulation:
     def write_vtu(self, prefix):
         self._check(self.lib.rkdg_write_vtu(self.ctx, str(prefix).encode()))
 
+    def write_vtk(self, prefix):
+        self._check(self.lib.rkdg_write_vtk(self.ctx, str(prefix).encode()))
+
     # ---- learned troubled-cell indicator
     def set_indicator(self, flat=None, threshold=0.0, judge=False):
         """Gate the limiter with the MLP of flat parameters (w1 b1 w2 b2 w3 b3 w4 b4); None removes it."""

@@ -164,6 +164,7 @@ SIGNATURES = {
     "rkdg_batch_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(Schedule), _dp]),
     "rkdg_write_modal": (C.c_int, [C.c_void_p, C.c_char_p]),
     "rkdg_write_vtu": (C.c_int, [C.c_void_p, C.c_char_p]),
+    "rkdg_write_vtk": (C.c_int, [C.c_void_p, C.c_char_p]),
     "rkdg_set_indicator": (C.c_int, [C.c_void_p, C.POINTER(MlpWeights), C.c_int32]),
     "rkdg_get_indicator_stats": (C.c_int, [C.c_void_p, C.POINTER(IndicatorStats), C.c_int32]),
     "rkdg_capture": (C.c_int, [C.c_void_p, C.c_int32]),

@@ -1508,7 +1508,9 @@ rkdg_status rkdg_get_capture(rkdg_ctx* ctx, float* features, uint8_t* label, uin
 
 // Writes the current state at a step boundary (the reference writes state[0] from
 // Writer::WriteOutput, simulation/writer.hpp:121-196).
-static rkdg_status write_output(rkdg_ctx* ctx, const char* prefix, bool vtu) {
+enum class OutputKind { modal, vtu, vtk };
+
+static rkdg_status write_output(rkdg_ctx* ctx, const char* prefix, OutputKind kind) {
     if (!ctx || !prefix) return RKDG_ERR_ARG;
     if (ctx->batch()) return set_err(ctx, RKDG_ERR_ARG, "batch context: write each instance from its own context");
     const rkdg::Model& M = ctx->model;
@@ -1519,14 +1521,18 @@ static rkdg_status write_output(rkdg_ctx* ctx, const char* prefix, bool vtu) {
     rkdg_status st = ctx->eng->get_state(q.data(), wet.data(), nullptr, nullptr);
     if (st != RKDG_OK) return set_err(ctx, st, ctx->eng->err);
     std::string err;
-    const bool ok = vtu ? rkdg::write_vtu(M, q.data(), wet.data(), (int)M.stepper.step, prefix, err)
-                        : rkdg::write_modal(M, q.data(), M.stepper.stage_time(), (int)M.stepper.step, prefix, err);
+    const int step = (int)M.stepper.step;
+    const bool ok = kind == OutputKind::vtu   ? rkdg::write_vtu(M, q.data(), wet.data(), step, prefix, err)
+                    : kind == OutputKind::vtk ? rkdg::write_vtk(M, q.data(), wet.data(), step, prefix, err)
+                                              : rkdg::write_modal(M, q.data(), M.stepper.stage_time(), step, prefix, err);
     return ok ? RKDG_OK : set_err(ctx, RKDG_ERR_ARG, err);
 }
 
-rkdg_status rkdg_write_modal(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, false); }
+rkdg_status rkdg_write_modal(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, OutputKind::modal); }
 
-rkdg_status rkdg_write_vtu(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, true); }
+rkdg_status rkdg_write_vtu(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, OutputKind::vtu); }
+
+rkdg_status rkdg_write_vtk(rkdg_ctx* ctx, const char* prefix) { return write_output(ctx, prefix, OutputKind::vtk); }
 
 rkdg_status rkdg_master_tables(int32_t p, int32_t* ndof, int32_t* ngp, double* phi, double* dz1, double* dz2,
                                double* w) {

@@ -16,6 +16,13 @@
 // wet flag.  went_completely_dry is set by the reference's wetting-drying pass when the mean
 // vertex depth is at or below h_o's threshold (swe_post_wet_dry.hpp:24-28), which leaves the
 // element at exactly that mean depth; it is written here as that condition on the state.
+//
+// VTK legacy (write_VTK_data, swe_post_write_vtk.hpp:5-95, after the geometry of
+// Writer::InitializeMeshGeometryVTK, simulation/writer.hpp:195-258, merged as WriteOutput does,
+// :121-151): <prefix>_data_<step>.vtk, ASCII, the same points, cells and values as the VTU (cell
+// data ze/qx/qy/bath, ID, wet_dry, went_completely_dry; point data ze/qx/qy/bath), in the
+// reference's layout and its iostream number formatting (six significant digits, values rounded
+// to float first).
 #include <array>
 #include <cmath>
 #include <cstdio>
@@ -45,7 +52,87 @@ double at(const double* u, size_t stride, int e, int nd, const double* phi) {
     return s;
 }
 
+// Post-processing values of one state: the modal expansions evaluated at the three vertices and
+// the centroid of each element (the reference's N_DIV = 1 subdivision).
+struct PostValues {
+    const Model& M;
+    const double* q;
+    int nd;
+    size_t ne;
+    std::vector<double> phi;  // [nd][4]: vertices 0-2, centroid 3
+    std::vector<double> pv;
+    PostValues(const Model& M_, const double* q_) : M(M_), q(q_), nd(M_.ndof), ne((size_t)M_.ne) {
+        const std::vector<std::array<double, 2>> pts = {{-1.0, -1.0}, {1.0, -1.0}, {-1.0, 1.0},
+                                                         {-1.0 / 3.0, -1.0 / 3.0}};
+        phi.resize((size_t)nd * 4);
+        dubiner_eval(M.p, pts, phi.data(), nullptr, nullptr);
+        pv.resize(nd);
+    }
+    const double* phi_at(int pt) {
+        for (int d = 0; d < nd; ++d) pv[d] = phi[(size_t)d * 4 + pt];
+        return pv.data();
+    }
+    double var(int v, int e, int pt) { return at(q + (size_t)v * nd * ne, ne, e, nd, phi_at(pt)); }
+    double bath(int e, int pt) { return at(M.bath.data(), ne, e, nd, phi_at(pt)); }
+    // mean vertex depth of the linear part (modes 0-2 at the vertices) + vertex bathymetry at or
+    // below the wetting-drying threshold
+    bool went_completely_dry(int e) {
+        double hsum = 0.0;
+        for (int pt = 0; pt < 3; ++pt) hsum += at(q, ne, e, nd < 3 ? nd : 3, phi_at(pt)) + M.bath_vrtx[(size_t)pt * ne + e];
+        return hsum / 3.0 <= M.h_thr;
+    }
+};
+
 }  // namespace
+
+bool write_vtk(const Model& M, const double* q, const uint8_t* wet, int step, const std::string& prefix,
+               std::string& err) {
+    const int ne = M.ne;
+    const std::string path = prefix + "_data_" + std::to_string(step) + ".vtk";
+    File fl(path, "w");
+    if (!fl.f) {
+        err = "cannot open " + path;
+        return false;
+    }
+    FILE* f = fl.f;
+    PostValues P(M, q);
+    std::fprintf(f, "# vtk DataFile Version 3.0\nOUTPUT DATA\nASCII\nDATASET UNSTRUCTURED_GRID\n");
+    std::fprintf(f, "POINTS %d double\n", 3 * ne);
+    for (int e = 0; e < ne; ++e)
+        for (int k = 0; k < 3; ++k) std::fprintf(f, "%g\t%g\t%g\n", M.vxy[(size_t)k * ne + e], M.vxy[(size_t)(3 + k) * ne + e], 0.0);
+    std::fprintf(f, "CELLS %d %d\n", ne, 4 * ne);
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "3\t%d\t%d\t%d\t\n", 3 * e, 3 * e + 1, 3 * e + 2);
+    std::fprintf(f, "CELL_TYPES %d\n", ne);
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "5\n");  // straight_triangle (general_definitions.hpp:242)
+    std::fprintf(f, "CELL_DATA %d\n", ne);
+    const char* vn[3] = {"ze", "qx", "qy"};
+    for (int v = 0; v < 3; ++v) {
+        std::fprintf(f, "SCALARS %s_cell double 1\nLOOKUP_TABLE default\n", vn[v]);
+        for (int e = 0; e < ne; ++e) std::fprintf(f, "%g\n", (double)(float)P.var(v, e, 3));
+    }
+    std::fprintf(f, "SCALARS bath_cell double 1\nLOOKUP_TABLE default\n");
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "%g\n", (double)(float)P.bath(e, 3));
+    std::fprintf(f, "SCALARS ID unsigned_int 1\nLOOKUP_TABLE default\n");
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "%d\n", e);
+    std::fprintf(f, "SCALARS wet_dry bit 1\nLOOKUP_TABLE default\n");
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "%d\n", wet[e] ? 1 : 0);
+    std::fprintf(f, "SCALARS went_completely_dry bit 1\nLOOKUP_TABLE default\n");
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "%d\n", P.went_completely_dry(e) ? 1 : 0);
+    std::fprintf(f, "POINT_DATA %d\n", 3 * ne);
+    for (int v = 0; v < 3; ++v) {
+        std::fprintf(f, "SCALARS %s_point double 1\nLOOKUP_TABLE default\n", vn[v]);
+        for (int e = 0; e < ne; ++e)
+            for (int pt = 0; pt < 3; ++pt) std::fprintf(f, "%g\n", (double)(float)P.var(v, e, pt));
+    }
+    std::fprintf(f, "SCALARS bath_point double 1\nLOOKUP_TABLE default\n");
+    for (int e = 0; e < ne; ++e)
+        for (int pt = 0; pt < 3; ++pt) std::fprintf(f, "%g\n", (double)(float)P.bath(e, pt));
+    if (std::ferror(f)) {
+        err = "write error on " + path;
+        return false;
+    }
+    return true;
+}
 
 bool write_modal(const Model& M, const double* q, double t, int step, const std::string& prefix, std::string& err) {
     const int ne = M.ne, nd = M.ndof;
@@ -73,7 +160,7 @@ bool write_modal(const Model& M, const double* q, double t, int step, const std:
 
 bool write_vtu(const Model& M, const double* q, const uint8_t* wet, int step, const std::string& prefix,
                std::string& err) {
-    const int ne = M.ne, nd = M.ndof;
+    const int ne = M.ne;
     const std::string path = prefix + "_data_" + std::to_string(step) + ".vtu";
     File fl(path, "w");
     if (!fl.f) {
@@ -81,18 +168,9 @@ bool write_vtu(const Model& M, const double* q, const uint8_t* wet, int step, co
         return false;
     }
     FILE* f = fl.f;
-    // basis values at the three vertices and the centroid of the master triangle
-    const std::vector<std::array<double, 2>> pts = {{-1.0, -1.0}, {1.0, -1.0}, {-1.0, 1.0}, {-1.0 / 3.0, -1.0 / 3.0}};
-    std::vector<double> phi((size_t)nd * 4);
-    dubiner_eval(M.p, pts, phi.data(), nullptr, nullptr);
-    std::vector<double> pv(nd);
-    auto phi_at = [&](int pt) {
-        for (int d = 0; d < nd; ++d) pv[d] = phi[(size_t)d * 4 + pt];
-        return pv.data();
-    };
-    const size_t sq = (size_t)ne;  // stride of a mode row
-    auto qv = [&](int var, int e, int pt) { return at(q + (size_t)var * nd * ne, sq, e, nd, phi_at(pt)); };
-    auto bv = [&](int e, int pt) { return at(M.bath.data(), sq, e, nd, phi_at(pt)); };
+    PostValues P(M, q);
+    auto qv = [&](int var, int e, int pt) { return P.var(var, e, pt); };
+    auto bv = [&](int e, int pt) { return P.bath(e, pt); };
 
     std::fprintf(f, "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\">\n\t<UnstructuredGrid>\n");
     std::fprintf(f, "\t\t<Piece NumberOfPoints=\"%d\" NumberOfCells=\"%d\">\n", 3 * ne, ne);
@@ -128,11 +206,7 @@ bool write_vtu(const Model& M, const double* q, const uint8_t* wet, int step, co
     for (int e = 0; e < ne; ++e) std::fprintf(f, "%d ", wet[e] ? 1 : 0);
     std::fprintf(f, "\n\t\t\t\t</DataArray>\n");
     std::fprintf(f, "\t\t\t\t<DataArray type=\"Int8\" Name=\"went_completely_dry\" format=\"ascii\">\n\t\t\t\t\t");
-    for (int e = 0; e < ne; ++e) {
-        double hsum = 0.0;  // mean vertex depth of the linear part (modes 0-2 at the vertices) + vertex bathymetry
-        for (int pt = 0; pt < 3; ++pt) hsum += at(q, sq, e, nd < 3 ? nd : 3, phi_at(pt)) + M.bath_vrtx[(size_t)pt * ne + e];
-        std::fprintf(f, "%d ", hsum / 3.0 <= M.h_thr ? 1 : 0);
-    }
+    for (int e = 0; e < ne; ++e) std::fprintf(f, "%d ", P.went_completely_dry(e) ? 1 : 0);
     std::fprintf(f, "\n\t\t\t\t</DataArray>\n\t\t\t</CellData>\n");
     // geometry (the reference's "foot")
     std::fprintf(f, "\t\t\t<Points>\n\t\t\t\t<DataArray type=\"Float64\" NumberOfComponents=\"3\" format=\"ascii\">\n\t\t\t\t\t");

@@ -18,8 +18,7 @@ Differences, each deliberate:
 - meteo forcing, tide potential and Coriolis are parsed and reported but have no effect, as in the
   reference, whose RKDG source term for them is commented out (swe_source.hpp:53-96);
 - relative file names resolve against `base_dir` (default: the current directory, as in the
-  reference);
-- VTK legacy output (output.vtk) is not written: the VTU writer covers the same point data.
+  reference).
 """
 import math
 import os
@@ -364,8 +363,6 @@ def read_input(path, problem=PROBLEM_NONE, problem_param=(0.0, 0.0, 0.0, 0.0), b
                 if step < 1:
                     raise ValueError(f"Error: {kind} output frequency must be positive\n")
                 setattr(output, kind + "_step", step)
-        if output.vtk_step:
-            warn.append("Note: VTK legacy output is not written by this engine; use vtu\n")
 
     if fmt == "Adcirc":
         mesh = read_adcirc(mesh_file, bcis_file if os.path.isfile(bcis_file) else None)
@@ -402,6 +399,8 @@ def run(inp, precision=RKDG_FP64_PLAIN, n_steps=None):
     def write(step):
         if out is None:
             return
+        if out.vtk_step and step % out.vtk_step == 0:
+            sim.write_vtk(prefix)
         if out.vtu_step and step % out.vtu_step == 0:
             sim.write_vtu(prefix)
         if out.modal_step and step % out.modal_step == 0:

@@ -390,9 +390,12 @@ rkdg_status rkdg_halo_unpack(rkdg_ctx* ctx, int32_t slot, const void* device_buf
  *   is the time, then "ID value" per element and mode.
  * rkdg_write_vtu: write_VTU_data + the geometry of Writer::InitializeMeshGeometryVTU
  *   (swe_post_write_vtu.hpp:5-103, simulation/writer.hpp:260-350): <prefix>_data_<step>.vtu.
+ * rkdg_write_vtk: write_VTK_data + the geometry of Writer::InitializeMeshGeometryVTK
+ *   (swe_post_write_vtk.hpp:5-95, simulation/writer.hpp:195-258): <prefix>_data_<step>.vtk.
  * prefix = output path + mesh name, as the reference composes it. */
 rkdg_status rkdg_write_modal(rkdg_ctx* ctx, const char* prefix);
 rkdg_status rkdg_write_vtu(rkdg_ctx* ctx, const char* prefix);
+rkdg_status rkdg_write_vtk(rkdg_ctx* ctx, const char* prefix);
 
 /* ---- learned troubled-cell indicator ------------------------------------------------
  * No counterpart in the reference.  A 36 -> 64 -> 64 -> 64 -> 1 ReLU MLP predicts, per element,

@@ -108,3 +108,44 @@ def test_output_at_step_boundary_only(tmp_path):
     s.write_vtu(str(tmp_path / "x"))
     assert (tmp_path / "x_data_1.vtu").exists()
     s.close()
+
+
+def test_vtk_legacy(tmp_path):
+    # the VTK legacy file carries the VTU's points, cells and values in the layout of
+    # Writer::InitializeMeshGeometryVTK + write_VTK_data (simulation/writer.hpp:195-258,
+    # swe_post_write_vtk.hpp:5-95): same numbers, same six-significant-digit printing
+    mesh, prm = W.bowl(scale=0.1)
+    s = Simulation(mesh, prm, RKDG_FP64_PLAIN)
+    s.step(3)
+    pre = str(tmp_path / "bowl")
+    s.write_vtu(pre)
+    s.write_vtk(pre)
+    ne = mesh.n_elements
+    piece = ET.parse(tmp_path / "bowl_data_3.vtu").getroot().find("UnstructuredGrid/Piece")
+    vtu = {d.get("Name") or "points": d.text.split() for d in piece.iter("DataArray")}
+    lines = (tmp_path / "bowl_data_3.vtk").read_text().split("\n")
+    assert lines[:5] == ["# vtk DataFile Version 3.0", "OUTPUT DATA", "ASCII", "DATASET UNSTRUCTURED_GRID",
+                         f"POINTS {3 * ne} double"]
+    pts = lines[5:5 + 3 * ne]
+    assert [t for l in pts for t in l.split("\t")] == vtu["points"]
+    i = 5 + 3 * ne
+    assert lines[i] == f"CELLS {ne} {4 * ne}"
+    assert lines[i + 1:i + 1 + ne] == [f"3\t{3 * e}\t{3 * e + 1}\t{3 * e + 2}\t" for e in range(ne)]
+    i += 1 + ne
+    assert lines[i] == f"CELL_TYPES {ne}" and lines[i + 1:i + 1 + ne] == ["5"] * ne
+    i += 1 + ne
+    assert lines[i] == f"CELL_DATA {ne}"
+    i += 1
+    kinds = {"ID": "unsigned_int", "wet_dry": "bit", "went_completely_dry": "bit"}
+    for name in ("ze_cell", "qx_cell", "qy_cell", "bath_cell", "ID", "wet_dry", "went_completely_dry"):
+        assert lines[i] == f"SCALARS {name} {kinds.get(name, 'double')} 1" and lines[i + 1] == "LOOKUP_TABLE default"
+        assert lines[i + 2:i + 2 + ne] == vtu[name], name
+        i += 2 + ne
+    assert lines[i] == f"POINT_DATA {3 * ne}"
+    i += 1
+    for name in ("ze_point", "qx_point", "qy_point", "bath_point"):
+        assert lines[i] == f"SCALARS {name} double 1" and lines[i + 1] == "LOOKUP_TABLE default"
+        assert lines[i + 2:i + 2 + 3 * ne] == vtu[name], name
+        i += 2 + 3 * ne
+    assert lines[i:] == [""]
+    s.close()

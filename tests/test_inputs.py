@@ -252,7 +252,7 @@ def test_run_weir_input_file_matches_a_direct_run(tmp_path):
     import yaml
     doc = yaml.safe_load((WEIR / "dgswemv2_input.15").read_text())
     doc["output"] = {"path": str(tmp_path / "out"), "vtu": {"frequency": 15}, "modal": {"frequency": 30},
-                     "logfile": {"name": "log", "verbose": True}}
+                     "vtk": {"frequency": 60}, "logfile": {"name": "log", "verbose": True}}
     (tmp_path / "in.15").write_text(yaml.safe_dump(doc))
     r = I.read_input(tmp_path / "in.15", base_dir=str(WEIR))
     assert (r.output.vtu_step, r.output.modal_step) == (5, 10)
@@ -264,6 +264,7 @@ def test_run_weir_input_file_matches_a_direct_run(tmp_path):
         assert np.array_equal(a[f], b[f]), f
     out = tmp_path / "out"
     assert sorted(x.name for x in out.glob("*.vtu")) == sorted(f"quadrilateral_data_{s}.vtu" for s in (0, 5, 10, 15, 20))
+    assert sorted(x.name for x in out.glob("*.vtk")) == sorted(f"quadrilateral_data_{s}.vtk" for s in (0, 20))
     ze = (out / "quadrilateral_modal_ze.txt").read_text().split("\n")
     assert (out / "log").read_text().startswith("Starting simulation with p=1 for quadrilateral mesh")
     assert sum(1 for line in ze if line and len(line.split()) == 1) == 3  # records at steps 0, 10, 20
