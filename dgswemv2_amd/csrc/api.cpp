@@ -249,14 +249,15 @@ struct Engine final : EngineBase {
         return e;
     }
     // host -> device: chunk c is converted into buffer c % 2 once that buffer's previous copy
-    // (chunk c - 2) has completed, then copied while chunk c + 1 is converted
+    // (chunk c - 2, or a copy of an earlier call still in flight) has completed, then copied
+    // while chunk c + 1 is converted; the copies are asynchronous on the context's stream
     hipError_t write_state(Real* dev, const double* in, size_t n) {
         hipError_t e = ensure_pinned();
         if (e != hipSuccess) return e;
         const size_t per = kPinBytes / sizeof(Real), nch = (n + per - 1) / per;
         for (size_t c = 0; c < nch && e == hipSuccess; ++c) {
             const size_t off = c * per, cnt = std::min(per, n - off);
-            if (c >= 2) e = hipEventSynchronize(pin_ev[c & 1]);
+            e = hipEventSynchronize(pin_ev[c & 1]);  // returns at once for an event never recorded
             Real* dst = static_cast<Real*>(pin[c & 1]);
             if (e == hipSuccess) {
                 for (size_t i = 0; i < cnt; ++i) dst[i] = (Real)in[off + i];
@@ -628,18 +629,13 @@ struct Engine final : EngineBase {
         if (M.stepper.stage != 0) return fail("instance access needs a step boundary", hipErrorInvalidValue), RKDG_ERR_ARG;
         const size_t base = M.ens.base[i], ne = M.ens.ne[i], rows = 3 * (size_t)M.ndof;
         hipError_t e = hipStreamSynchronize(stream);
-        std::vector<Real> tmp(rows * ne);
-        if (e == hipSuccess && (qh || qin)) {
-            if (qin) {
-                for (size_t k = 0; k < tmp.size(); ++k) tmp[k] = (Real)qin[k];
-                e = hipMemcpy2D(q[0] + base, M.ne * sizeof(Real), tmp.data(), ne * sizeof(Real), ne * sizeof(Real), rows,
-                                hipMemcpyHostToDevice);
-            } else {
-                e = hipMemcpy2D(tmp.data(), ne * sizeof(Real), q[0] + base, M.ne * sizeof(Real), ne * sizeof(Real), rows,
-                                hipMemcpyDeviceToHost);
-                for (size_t k = 0; k < tmp.size() && e == hipSuccess; ++k) qh[k] = (double)tmp[k];
-            }
+        // each mode row of the instance is a contiguous run of ne values at `base` in a row of the
+        // merged range: moved through the pinned staging buffers row by row
+        for (size_t r = 0; r < rows && e == hipSuccess && (qh || qin); ++r) {
+            Real* dev = q[0] + r * (size_t)M.ne + base;
+            e = qin ? write_state(dev, qin + r * ne, ne) : read_state(dev, qh + r * ne, ne);
         }
+        if (e == hipSuccess && qin) e = hipStreamSynchronize(stream);
         if (e == hipSuccess && win) e = hipMemcpy(wet[wcur] + base, win, ne, hipMemcpyHostToDevice);
         if (e == hipSuccess && wh) e = hipMemcpy(wh, wet[wcur] + base, ne, hipMemcpyDeviceToHost);
         if (e != hipSuccess) return fail("instance copy", e), RKDG_ERR_HIP;
