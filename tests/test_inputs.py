@@ -269,3 +269,38 @@ def test_run_weir_input_file_matches_a_direct_run(tmp_path):
     assert (out / "log").read_text().startswith("Starting simulation with p=1 for quadrilateral mesh")
     assert sum(1 for line in ze if line and len(line.split()) == 1) == 3  # records at steps 0, 10, 20
     assert math.isfinite(float(ze[0]))
+
+
+def test_meta_format_input(tmp_path):
+    # format: Meta reads a MeshMetaData file (mesh_metadata.cpp:87-137) with the .bcis beside it
+    import shutil
+    import yaml
+    from dgswemv2_amd import read_meta, write_meta
+    m = read_adcirc(WEIR / "weir.14", WEIR / "weir.bcis")
+    write_meta(m, tmp_path / "weir.meta", name="weirmeta")
+    shutil.copy(WEIR / "weir.bcis", tmp_path / "weir.bcis")
+    doc = yaml.safe_load((WEIR / "dgswemv2_input.15").read_text())
+    doc["mesh"].update(format="Meta", file_name="weir.meta")
+    (tmp_path / "in.15").write_text(yaml.safe_dump(doc))
+    r = I.read_input(tmp_path / "in.15", base_dir=str(tmp_path))
+    ref = read_meta(tmp_path / "weir.meta", tmp_path / "weir.bcis")
+    assert r.mesh_name == "weirmeta"
+    for f in ("node_xyz", "elt_nodes", "bound_edges", "bound_types", "levee_nodes", "levee_param"):
+        assert np.array_equal(getattr(r.mesh, f), getattr(ref, f)), f
+
+
+def test_spherical_coordinates_input(tmp_path):
+    import yaml
+    doc = yaml.safe_load((WEIR / "dgswemv2_input.15").read_text())
+    doc["mesh"]["coordinate_system"] = "spherical"
+    (tmp_path / "a.15").write_text(yaml.safe_dump(doc))
+    r = I.read_input(tmp_path / "a.15", base_dir=str(WEIR))
+    # without a projection block: projected with the defaults, correction off, the reference's warning
+    assert r.params.spherical_mesh == 1 and r.params.sp_correction == 0 and r.params.sph_R == 6378200.0
+    assert any("Spherical projection input not provided" in w for w in r.warnings)
+    doc["problem"]["spherical_projection"] = {"longitude_o": 10.0, "latitude_o": 45.0, "R": 6.4e6}
+    (tmp_path / "b.15").write_text(yaml.safe_dump(doc))
+    r = I.read_input(tmp_path / "b.15", base_dir=str(WEIR))
+    p = r.params
+    assert p.spherical_mesh == 1 and p.sp_correction == 1 and (p.sph_lon_o, p.sph_lat_o, p.sph_R) == (10.0, 45.0, 6.4e6)
+    assert r.warnings == []
