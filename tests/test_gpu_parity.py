@@ -246,8 +246,11 @@ def test_full_size_tidal_fp64_fast_vs_plain():
     ("manufactured", dict(scale=0.25), 20, 1e-5),
     ("tidal", dict(scale=0.04), 10, 1e-5),
     ("tidal", dict(scale=0.05, p=3), 10, 1e-5),
+    ("tidal", dict(scale=0.04, nstages=8, order=4), 6, 1e-5),
+    ("tidal", dict(scale=0.04, nstages=1, order=1), 6, 1e-5),
 ])
 def test_fp32_bulk_against_fp64_oracle(name, kw, steps, tol):
+    register_ssprk_schemes()
     mesh, prm = W.WORKLOADS[name](**kw)
     o = OracleSim(mesh, prm)
     g = Simulation(mesh, prm, RKDG_FP32_BULK)
