@@ -29,7 +29,7 @@ import yaml
 from . import (FRICTION_CHEZY, FRICTION_MANNING, PROBLEM_NONE, RKDG_FP64_PLAIN, Simulation, make_params, read_adcirc,
                read_meta)
 
-__all__ = ["OutputSpec", "RunInput", "read_input", "run", "run_time_seconds"]
+__all__ = ["OutputSpec", "RunInput", "read_input", "run", "run_time_seconds", "swe_inputs", "write_input"]
 
 
 @dataclass
@@ -54,6 +54,7 @@ class RunInput:
     output: OutputSpec = None
     warnings: list = field(default_factory=list)
     unused: dict = field(default_factory=dict)  # parsed, without effect on the RKDG stage
+    source: dict = field(default_factory=dict)  # names and values as the file gave them (write_input)
 
 
 # ---- dates: strptime(s, "%d-%m-%Y %H:%M:%S", &tm) into a zeroed tm, then timegm
@@ -341,12 +342,16 @@ def read_input(path, problem=PROBLEM_NONE, problem_param=(0.0, 0.0, 0.0, 0.0), b
             raise ValueError("Error: Load Balancer YAML node is malformatted\n")
         unused["load_balancer"] = {"name": str(lb["name"]), "rebalance_frequency": _num(lb, "rebalance_frequency")}
 
+    source = {"mesh": {"format": fmt, "file_name": str(mesh_node["file_name"]), "coordinate_system": coord},
+              "start_time": run_time_seconds(ts["start_time"]), "end_time": run_time_seconds(ts["end_time"]),
+              "manning_file": str(pr["bottom_friction"]["input_file"]) if manning_file else None, "frequency": {}}
     output = None
     if "output" in doc:
         out = doc["output"]
+        source["output_path"] = opath_given = str(out["path"]) if _present(out, "path") else None
         if not _present(out, "path"):
             raise ValueError("Error: Output YAML node is malformatted\n")
-        opath = str(out["path"])
+        opath = opath_given
         output = OutputSpec(path=_resolve(opath if opath.endswith("/") else opath + "/", base_dir))
         if "logfile" in out:
             lf = out["logfile"]
@@ -359,6 +364,7 @@ def read_input(path, problem=PROBLEM_NONE, problem_param=(0.0, 0.0, 0.0, 0.0), b
                 if not _present(node, "frequency"):
                     raise ValueError(f"Error: {kind.upper() if kind != 'modal' else 'Modal'} YAML node is "
                                      "malformatted\n")
+                source["frequency"][kind] = _num(node, "frequency")
                 step = math.ceil(_num(node, "frequency") / dt)
                 if step < 1:
                     raise ValueError(f"Error: {kind} output frequency must be positive\n")
@@ -377,7 +383,59 @@ def read_input(path, problem=PROBLEM_NONE, problem_param=(0.0, 0.0, 0.0, 0.0), b
     params = make_params(**kw)
     n_steps = math.ceil(run_time / dt) if dt > 0 else 0
     return RunInput(mesh=mesh, params=params, mesh_name=mesh_name, run_time=run_time, n_steps=n_steps,
-                    output=output, warnings=warn, unused=unused)
+                    output=output, warnings=warn, unused=unused, source=source)
+
+
+def write_input(inp, path):
+    """Writes a RunInput as a YAML input file (InputParameters::write_to, input_parameters.hpp:
+    296-391, with SWE::Inputs::as_yaml_node, swe_inputs.cpp:316-431): the run parameters as read,
+    the file names as the input gave them, the dates normalised as timegm leaves them; reading the
+    written file gives the same run."""
+    import time
+    p, src, un = inp.params, inp.source, inp.unused
+
+    def date(secs):
+        return time.strftime("%d-%m-%Y %H:%M:%S", time.gmtime(secs))
+
+    prob = {"name": "rkdg_swe", "gravity": p.g, "density_air": un.get("density_air", 1.2250),
+            "density_water": un.get("density_water", 1000.0)}
+    if p.sp_correction:
+        prob["spherical_projection"] = {"type": "Enable", "longitude_o": p.sph_lon_o, "latitude_o": p.sph_lat_o,
+                                        "R": p.sph_R}
+    if p.ic_constant:
+        prob["initial_conditions"] = {"type": "Constant", "initial_surface_height": p.ze0, "initial_momentum_x": p.qx0,
+                                      "initial_momentum_y": p.qy0}
+    else:
+        prob["initial_conditions"] = {"type": "Function"}
+    if p.function_source:
+        prob["function_source"] = "Enable"
+    if p.friction == FRICTION_CHEZY:
+        prob["bottom_friction"] = {"type": "Chezy", "coefficient": p.friction_cf}
+    elif p.friction == FRICTION_MANNING:
+        prob["bottom_friction"] = {"type": "Manning", "coefficient": p.friction_cf, "input_file": src["manning_file"]}
+    for key in ("meteo_forcing", "tide_potential", "coriolis"):
+        if key in un:
+            prob[key] = {"type": un[key]} if key == "tide_potential" else un[key]
+    if p.wetting_drying:
+        prob["wetting_drying"] = {"h_o": p.h_o}
+    if p.slope_limiting:
+        prob["slope_limiting"] = {"type": "Cockburn-Shu", "M": p.sl_M, "nu": p.sl_nu}
+    doc = {"polynomial_order": int(p.p), "mesh": dict(src["mesh"]),
+           "timestepping": {"start_time": date(src["start_time"]), "end_time": date(src["end_time"]), "dt": p.dt,
+                            "order": int(p.order), "nstages": int(p.nstages), "ramp_duration": p.ramp_duration},
+           "problem": prob}
+    if inp.output is not None:
+        out = {"path": src["output_path"]}
+        if inp.output.log_name:
+            out["logfile"] = {"name": inp.output.log_name, "verbose": bool(inp.output.log_verbose)}
+        for kind, f in src["frequency"].items():
+            out[kind] = {"frequency": f}
+        doc["output"] = out
+    if "load_balancer" in un:
+        doc["load_balancer"] = dict(un["load_balancer"])
+    with open(path, "w") as f:
+        f.write("#" * 79 + "\n#\n#  DG_HYPER input file\n#\n" + "#" * 79 + "\n\n")
+        f.write(yaml.safe_dump(doc, sort_keys=False))
 
 
 def run(inp, precision=RKDG_FP64_PLAIN, n_steps=None):

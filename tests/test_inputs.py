@@ -304,3 +304,31 @@ def test_spherical_coordinates_input(tmp_path):
     p = r.params
     assert p.spherical_mesh == 1 and p.sp_correction == 1 and (p.sph_lon_o, p.sph_lat_o, p.sph_R) == (10.0, 45.0, 6.4e6)
     assert r.warnings == []
+
+
+def _params_equal(a, b):
+    return all(getattr(a, f) == getattr(b, f) if f != "problem_param" else list(a.problem_param) == list(b.problem_param)
+               for f, _ in type(a)._fields_)
+
+
+@pytest.mark.parametrize("which", ["correct", "weir"])
+def test_written_input_reads_back_the_same(tmp_path, which):
+    # test_input_parameters.cpp: read a correct file, write it (write_to), read the written file,
+    # and the two inputs are equal
+    if which == "correct":
+        (tmp_path / "in.15").write_text(CORRECT_INPUT.format(fort14=GOLDEN / "sample_fort.14"))
+        (tmp_path / "manning.test").write_text("".join(f"{i} 0.02\n" for i in range(1, 131)))
+        kw = dict(problem=PROBLEM_MANUFACTURED, base_dir=str(tmp_path))
+        src = tmp_path / "in.15"
+    else:
+        kw = dict(base_dir=str(WEIR))
+        src = WEIR / "dgswemv2_input.15"
+    a = I.read_input(src, **kw)
+    I.write_input(a, tmp_path / "emitted.15")
+    b = I.read_input(tmp_path / "emitted.15", **kw)
+    assert _params_equal(a.params, b.params)
+    assert (a.run_time, a.n_steps, a.mesh_name, a.output, a.unused, a.source) == \
+        (b.run_time, b.n_steps, b.mesh_name, b.output, b.unused, b.source)
+    for f in ("node_xyz", "elt_nodes", "bound_types"):
+        assert np.array_equal(getattr(a.mesh, f), getattr(b.mesh, f))
+    assert (tmp_path / "emitted.15").read_text().startswith("#" * 79)
