@@ -473,8 +473,8 @@ def test_bowl_full_size_300_steps():
 
 # ---- forcing ramp (ESSPRKStepper::GetRamp, explicit_ssp_rk_stepper.hpp:60-62)
 
-def _ramped_tidal(scale, p, seed, ramp_days):
-    mesh, prm = W.tidal(scale=scale, p=p, seed=seed)
+def _ramped_tidal(scale, p, seed, ramp_days, nstages=3, order=3):
+    mesh, prm = W.tidal(scale=scale, p=p, seed=seed, nstages=nstages, order=order)
     prm.ramp_duration = ramp_days
     return mesh, prm
 
@@ -501,14 +501,17 @@ def test_tidal_ramp_single_vs_oracle():
     banner("tidal ramp bit-exact", bool(np.array_equal(sg["q"], so["q"])))
 
 
-@pytest.mark.parametrize("graph", [0, 1])
-def test_tidal_ramp_batch_vs_oracle(graph):
+@pytest.mark.parametrize("graph,nstages,order", [(0, 3, 3), (1, 3, 3), (1, 5, 4), (0, 8, 4)])
+def test_tidal_ramp_batch_vs_oracle(graph, nstages, order):
     """Ensemble instances with different ramp durations (one without a ramp) and orders, stepped in
     one batch (eager stages or graph replay): each follows the oracle's run of its mesh per step.
     The ramp of each stage comes from the host's stepper arithmetic, so instances match their
-    single-mesh runs bit for bit."""
+    single-mesh runs bit for bit; the five- and eight-stage schemes fill the ramp table's wider
+    rows."""
     from dgswemv2_amd import Batch
-    cases = [_ramped_tidal(0.03, 1, 3, 0.0), _ramped_tidal(0.04, 2, 5, 2e-5), _ramped_tidal(0.03, 2, 7, 5e-5)]
+    register_ssprk_schemes()
+    cases = [_ramped_tidal(0.03, 1, 3, 0.0, nstages, order), _ramped_tidal(0.04, 2, 5, 2e-5, nstages, order),
+             _ramped_tidal(0.03, 2, 7, 5e-5, nstages, order)]
     b = Batch([c[0] for c in cases], [c[1] for c in cases], RKDG_FP64_PLAIN)
     b.set_schedule(block=128, groups_per_xcd=0, graph=graph)
     orc = [OracleSim(m, p) for m, p in cases]
@@ -522,7 +525,7 @@ def test_tidal_ramp_batch_vs_oracle(graph):
             assert r <= TOL64, (graph, i, k, r)
             assert np.array_equal(got, s.state()["q"]), (graph, i, k)
             assert b.time(k)[:2] == o.time()[:2]
-    banner("ramped ensemble graph", graph, "matches oracle and single runs")
+    banner("ramped ensemble graph", graph, "stages", nstages, "matches oracle and single runs")
     b.close()
 
 
